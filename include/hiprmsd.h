@@ -1,0 +1,163 @@
+/*
+ * hiprmsd — C ABI of the MI355X (gfx950) RMSD library.
+ *
+ * The reference ships its RMSD core as a C ABI over host pointers, one frame
+ * pair per call (rmsd/include/theobald_rmsd.h:7-18, rmsd/include/center.h:7-8,
+ * rmsd/include/rotation.h:7-12, installed as libtheobald.a via
+ * mdtraj/__init__.py:217-225).  This library replaces it with one call per
+ * BATCH over device pointers: every pointer argument below is device memory
+ * (hipMalloc'd or a torch CUDA tensor's data_ptr()) unless noted, every call
+ * is asynchronous on the given HIP stream (NULL = the default stream), and
+ * every entry point returns HR_OK (0) or an error code, with a message from
+ * hr_last_error() (thread-local).  Nothing is printed by the library.
+ *
+ * Coordinates are float32, atom-major ("AoS", shape (n_frames, n_atoms, 3),
+ * C order) exactly as mdtraj's Trajectory.xyz, or axis-major where stated.
+ * Atom selections are int32 device arrays (NULL = all atoms).
+ *
+ * Precision: per-frame sums (centroid, trace, correlation matrix) are
+ * accumulated in FP64 on the GPU and the QCP eigenvalue and the final
+ * MSD = (G_a + G_b - 2 lambda) / N are evaluated in FP64; results are
+ * rounded to float32 once at the end.
+ */
+#ifndef HIPRMSD_H_
+#define HIPRMSD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HR_OK 0
+#define HR_ERR_INVALID 1     /* bad argument (size, pointer, index range) */
+#define HR_ERR_HIP 2         /* a HIP runtime call failed */
+#define HR_ERR_UNSUPPORTED 3 /* configuration this build does not handle */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int hr_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hr_last_error(void);
+
+/* Select `device` for this thread and cache its properties (CU count, LDS
+ * size).  Optional: every entry point initialises the current device lazily. */
+int hr_init(int device);
+
+/* Number of compute units of the current device (after hr_init). */
+int hr_device_cu_count(void);
+
+/* ---------------------------------------------------------------------------
+ * Centering (replaces inplace_center_and_trace_atom_major,
+ * rmsd/include/center.h:7-8; rmsd/src/center_generic.h:3-44).
+ * In place on xyz (n_frames, n_atoms, 3): per frame, mean accumulated in
+ * double and cast to float, subtracted in float; traces[f] = sum of squares of
+ * the centred float coordinates accumulated in double, cast to float.
+ * traces may be NULL.
+ * ------------------------------------------------------------------------- */
+int hr_center_and_trace(float* xyz, float* traces, int64_t n_frames, int32_t n_atoms,
+                        void* stream);
+
+/* ---------------------------------------------------------------------------
+ * One-vs-all RMSD (replaces the frame loop of md.rmsd, rmsd/_rmsd.pyx:186-224,
+ * which calls msd_atom_major (rmsd/include/theobald_rmsd.h:16-18) per frame
+ * after inplace_center_and_trace_atom_major on copies of the data).
+ *
+ *   target      (n_frames, n_atoms_total, 3) float32, not modified
+ *   atom_idx    int32[n_sel] into the target's atoms, or NULL (all atoms,
+ *               then n_sel must equal n_atoms_total)
+ *   ref_frame   (n_ref_atoms_total, 3) float32: the reference conformation
+ *   ref_idx     int32[n_sel] into the reference's atoms, or NULL (then the
+ *               reference has n_sel atoms used in order)
+ *   self_frame  index of a target frame that IS the reference frame (same
+ *               buffer, same selection): its RMSD is written as exactly 0, as
+ *               the reference's pointer-equality early-out does
+ *               (rmsd/src/theobald_rmsd_sse.h:256-262); -1 for none
+ *   out         float32[n_frames] RMSDs in nm
+ *   workspace   device buffer of hr_rmsd_workspace_bytes(n_sel) bytes
+ *
+ * Targets are not centred in place (the reference mutates its inputs; this
+ * library never does).
+ * ------------------------------------------------------------------------- */
+size_t hr_rmsd_workspace_bytes(int32_t n_sel);
+int hr_rmsd_one_vs_all(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                       const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                       const int32_t* ref_idx, int64_t self_frame, float* out, void* workspace,
+                       void* stream);
+
+/* The two halves of hr_rmsd_one_vs_all, for callers that score many target
+ * batches against one reference: prepare gathers the reference selection into
+ * `workspace` and sums it in FP64 (one small kernel); the prepared call then
+ * runs the per-frame kernel only, reading the reference from `workspace`. */
+int hr_rmsd_prepare_reference(const float* ref_frame, const int32_t* ref_idx, int32_t n_sel,
+                              void* workspace, void* stream);
+int hr_rmsd_one_vs_all_prepared(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                                const int32_t* atom_idx, int32_t n_sel, int64_t self_frame,
+                                float* out, const void* workspace, void* stream);
+
+/* Same, for axis-major targets (n_frames, 3, rowstride) — replaces
+ * msd_axis_major (rmsd/include/theobald_rmsd.h:13-14) as driven by
+ * getMultipleRMSDs_axis_major (rmsd/_rmsd.pyx:500-557).  ref_frame_axis is
+ * (3, ref_rowstride).  No selection: the first n_atoms of each row are used. */
+int hr_rmsd_one_vs_all_axis_major(const float* target, int64_t n_frames, int32_t n_atoms,
+                                  int32_t rowstride, const float* ref_frame_axis,
+                                  int32_t ref_rowstride, int64_t self_frame, float* out,
+                                  void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * RMSD without superposition (md.rmsd(..., superpose=False);
+ * rmsd/_rmsd.pyx:228-241,765-793): sqrt(mean |a_i - b_i|^2), no centering,
+ * accumulated in FP64.  Arguments as hr_rmsd_one_vs_all (workspace of
+ * hr_rmsd_workspace_bytes(n_sel) bytes).
+ * ------------------------------------------------------------------------- */
+int hr_rmsd_nosuperpose(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                        const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                        const int32_t* ref_idx, float* out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Superposition (replaces Trajectory.superpose, core/trajectory.py:1083-1173,
+ * and the low-level superpose_atom_major, rmsd/_rmsd.pyx:620-674, which call
+ * msd_atom_major(..., computeRot=1) and rot_atom_major,
+ * rmsd/include/rotation.h:7).
+ *   xyz         (n_frames, n_atoms, 3) float32, moved in place
+ *   align_xyz   (n_frames, n_align_atoms, 3) frames the fit is computed on,
+ *               or NULL to fit on xyz itself (then n_align_atoms = n_atoms)
+ *   atom_idx    int32[n_sel] into align_xyz's atoms, or NULL (all)
+ *   ref_frame   (n_ref_atoms, 3), ref_idx int32[n_sel] or NULL
+ *   translate   1: x' = (x - c_f) R + c_ref with c_f the centroid of frame f's
+ *               fitted subset and c_ref the reference subset's centroid
+ *               (Trajectory.superpose); 0: x' = x R (superpose_atom_major,
+ *               which expects centred inputs)
+ *   rot_out     float32 (n_frames, 3, 3) row-major, applied as x' = x R, or NULL
+ *   rmsd_out    float32[n_frames] fit RMSDs, or NULL
+ *   workspace   hr_superpose_workspace_bytes(n_frames, n_sel) bytes
+ * ------------------------------------------------------------------------- */
+size_t hr_superpose_workspace_bytes(int64_t n_frames, int32_t n_sel);
+int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* align_xyz,
+                 int32_t n_align_atoms, const int32_t* atom_idx, int32_t n_sel,
+                 const float* ref_frame, const int32_t* ref_idx, int32_t translate,
+                 float* rot_out, float* rmsd_out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * All-pairs RMSD matrix.  The reference has no such routine; its users loop
+ * md.rmsd(traj, traj, i) over i (examples/clustering.ipynb:80-81), so row i
+ * of this matrix equals one-vs-all with reference frame i.
+ *   mode        HR_PAIRS_FP32  plain FP32 FMA kernel
+ *               HR_PAIRS_F16X3 FP16 hi+lo split, three MFMA products, FP32 accumulate
+ *               HR_PAIRS_BF16X3 BF16 hi+lo split, three MFMA products
+ *   out         float32 condensed upper triangle (i < j, row-major, the layout
+ *               of scipy.spatial.distance.squareform), n_frames*(n_frames-1)/2
+ * ------------------------------------------------------------------------- */
+#define HR_PAIRS_FP32 0
+#define HR_PAIRS_F16X3 1
+#define HR_PAIRS_BF16X3 2
+size_t hr_rmsd_all_pairs_workspace_bytes(int64_t n_frames, int32_t n_sel, int32_t mode);
+int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
+                      const int32_t* atom_idx, int32_t n_sel, int32_t mode, float* out,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPRMSD_H_ */

@@ -1,0 +1,88 @@
+"""ctypes binding of ``libhiprmsd.so`` (the C ABI declared in include/hiprmsd.h).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+every call raises ``RuntimeError`` — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhiprmsd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hiprmsd.h")
+
+HR_OK = 0
+HR_PAIRS_FP32 = 0
+HR_PAIRS_F16X3 = 1
+HR_PAIRS_BF16X3 = 2
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "hr_version": (ctypes.c_int, []),
+    "hr_last_error": (ctypes.c_char_p, []),
+    "hr_init": (ctypes.c_int, [ctypes.c_int]),
+    "hr_device_cu_count": (ctypes.c_int, []),
+    "hr_center_and_trace": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp]),
+    "hr_rmsd_workspace_bytes": (_sz, [_i32]),
+    "hr_rmsd_one_vs_all": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "hr_rmsd_prepare_reference": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),
+    "hr_rmsd_one_vs_all_prepared": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i64, _vp, _vp, _vp]),
+    "hr_rmsd_one_vs_all_axis_major": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _i32, _i64, _vp, _vp, _vp]),
+    "hr_rmsd_nosuperpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "hr_superpose_workspace_bytes": (_sz, [_i64, _i32]),
+    "hr_superpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "hr_rmsd_all_pairs_workspace_bytes": (_sz, [_i64, _i32, _i32]),
+    "hr_rmsd_all_pairs": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _sz, _vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class HipRmsdError(RuntimeError):
+    """A hiprmsd entry point returned an error status."""
+
+
+def exported_symbols():
+    """Names the C ABI declares (used by the ABI test)."""
+    return list(_SIGNATURES)
+
+
+def load():
+    """Load the library once; RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built: run `python -m mdtraj_amd._build` "
+                    "(hipcc, gfx950). There is no CPU fallback.")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != HR_OK:
+        msg = load().hr_last_error()
+        msg = msg.decode() if msg else ""
+        raise HipRmsdError(f"{what} failed (status {status}): {msg}")
+
+
+def call(name: str, *args):
+    """Call entry point `name` and raise on a non-zero status."""
+    fn = getattr(load(), name)
+    check(fn(*args), name)

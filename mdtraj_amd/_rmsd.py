@@ -1,0 +1,257 @@
+"""The ``mdtraj._rmsd`` surface on the MI355X kernels.
+
+Mirrors ``mdtraj/rmsd/_rmsd.pyx`` (paths relative to the mdtraj source tree):
+same names, argument meanings, defaults, validation order, exception types
+and messages.  Every computation runs in ``libhiprmsd.so`` on the GPU.
+
+Documented deviations from the reference:
+* inputs are never modified: the reference centres ``target.xyz`` in place
+  when ``atom_indices`` is None (``_rmsd.pyx:197,212``); here centering is
+  fused into the RMSD kernel and nothing is written back (so read-only and
+  memory-mapped arrays work, where the reference raises ``ValueError``);
+* ``precentered=True`` is accepted with the reference's warnings, but the
+  cached float32 traces are not used for arithmetic: centroids and traces are
+  recomputed in FP64 inside the same single pass, which costs nothing extra
+  and keeps the 1e-5 nm accuracy;
+* the reference frame's RMSD to itself is exactly 0 whenever the target IS
+  the reference and the selections agree (the reference gets 0 only through a
+  pointer-equality early-out, ``rmsd/src/theobald_rmsd_sse.h:256-262``);
+* ``parallel`` is accepted and ignored (the GPU path is always parallel);
+* when the two trajectories have different atom counts and no selection is
+  given, ``ValueError`` is raised (the reference's check there calls
+  ``len(slice)`` and raises ``TypeError``).
+* device-resident inputs (torch CUDA tensors as ``.xyz``) return a CUDA tensor.
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+from . import _device as dev
+from ._lib import call, load
+
+
+def _xyz_of(obj):
+    return obj.xyz if hasattr(obj, "xyz") else obj
+
+
+def _check_indices(idx, n_atoms, name):
+    """ensure_type(np.asarray(idx), dtype=int, ndim=1) plus the range check of
+    ``_rmsd.pyx:158-161,171-174``."""
+    if dev.is_device_array(idx):
+        a = idx
+        if a.dim() != 1:
+            raise ValueError(f"{name} must be ndim 1. You supplied {a.dim()}")
+        if a.numel() and (int(a.min()) < 0 or int(a.max()) >= n_atoms):
+            raise ValueError(f"{name} must be valid positive indices")
+        return a
+    a = np.asarray(idx)
+    if a.ndim != 1:
+        raise ValueError(f"{name} must be ndim 1. You supplied {a.ndim}")
+    a = a.astype(np.int64)
+    if not np.all((a >= 0) * (a < n_atoms)):
+        raise ValueError(f"{name} must be valid positive indices")
+    return a
+
+
+def _same_buffer(a, b) -> bool:
+    if a is b:
+        return True
+    if dev.is_device_array(a) and dev.is_device_array(b):
+        return a.data_ptr() == b.data_ptr() and tuple(a.shape) == tuple(b.shape)
+    if isinstance(a, np.ndarray) and isinstance(b, np.ndarray):
+        return (a.__array_interface__["data"][0] == b.__array_interface__["data"][0]
+                and a.shape == b.shape and a.strides == b.strides)
+    return False
+
+
+def _validate(target_xyz, reference_xyz, frame, atom_indices, ref_atom_indices):
+    """Validation of ``_rmsd.pyx:154-184``; returns (idx, ref_idx, n_sel, frame)."""
+    if atom_indices is not None:
+        atom_indices = _check_indices(atom_indices, target_xyz.shape[1], "atom_indices")
+    if ref_atom_indices is None:
+        ref_atom_indices = atom_indices
+    else:
+        if atom_indices is None:
+            raise TypeError("ref_atom_indices requires atom_indices "
+                            "(the reference takes len() of the default selection)")
+        if len(ref_atom_indices) != len(atom_indices):
+            raise ValueError(
+                "atom_indices and ref_atom_indices must have same number of atom indices. "
+                "found %d and %d." % (len(atom_indices), len(ref_atom_indices)))
+    if ref_atom_indices is not None and ref_atom_indices is not atom_indices:
+        ref_atom_indices = _check_indices(ref_atom_indices, reference_xyz.shape[1],
+                                          "ref_atom_indices")
+    assert (len(target_xyz.shape) == 3 and len(reference_xyz.shape) == 3
+            and target_xyz.shape[2] == 3 and reference_xyz.shape[2] == 3)
+    if atom_indices is None and target_xyz.shape[1] != reference_xyz.shape[1]:
+        raise ValueError("Input trajectories must have same number of atoms. "
+                         "found %d and %d." % (target_xyz.shape[1], reference_xyz.shape[1]))
+    n_ref_frames = reference_xyz.shape[0]
+    if frame >= n_ref_frames:
+        raise ValueError("Cannot calculate RMSD of frame %d: reference has "
+                         "only %d frames." % (frame, n_ref_frames))
+    if frame < -n_ref_frames:
+        raise IndexError("index %d is out of bounds for axis 0 with size %d"
+                         % (frame, n_ref_frames))
+    if frame < 0:
+        frame += n_ref_frames
+    n_sel = target_xyz.shape[1] if atom_indices is None else len(atom_indices)
+    return atom_indices, ref_atom_indices, n_sel, frame
+
+
+def _result(out, like_device):
+    return out if like_device else dev.to_host(out)
+
+
+def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
+         parallel=True, precentered=False, superpose=True):
+    """RMSD of every conformation in ``target`` to ``reference[frame]``.
+
+    Drop-in for ``md.rmsd`` (``rmsd/_rmsd.pyx:65-243``); see the module
+    docstring for the deviations.  Returns float32 (n_frames,) in nm — a numpy
+    array, or a CUDA tensor when ``target.xyz`` is one.
+    """
+    txyz = _xyz_of(target)
+    rxyz = _xyz_of(reference)
+    idx, ridx, n_sel, frame = _validate(txyz, rxyz, frame, atom_indices, ref_atom_indices)
+    if superpose:
+        # _rmsd.pyx:203-211: the cached traces are usable only with both
+        # caches present and no selection; otherwise the reference warns.
+        usable = (getattr(reference, "_rmsd_traces", None) is not None
+                  and getattr(target, "_rmsd_traces", None) is not None
+                  and atom_indices is None)
+        if precentered and not usable:
+            warnings.warn("in rmsd(), precentered is ignored when atom_indices != None",
+                          RuntimeWarning)
+    elif precentered:
+        warnings.warn("in rmsd(), precentered is ignored when superpose=False", RuntimeWarning)
+
+    lib = load()
+    on_device = dev.is_device_array(txyz)
+    same = _same_buffer(txyz, rxyz) and ref_atom_indices is None
+    T = dev.to_device_f32(txyz)
+    if same:
+        R = T[frame]
+    else:
+        R = dev.to_device_f32(rxyz[frame])
+    n_frames, n_atoms = int(T.shape[0]), int(T.shape[1])
+    out = dev.empty_f32(n_frames)
+    if n_frames == 0:
+        return _result(out, on_device)
+    d_idx = dev.to_device_i32(idx)
+    d_ridx = dev.to_device_i32(ridx)
+    ws = dev.workspace(lib.hr_rmsd_workspace_bytes(n_sel))
+    if superpose:
+        call("hr_rmsd_one_vs_all", T.data_ptr(), n_frames, n_atoms, dev.ptr(d_idx), n_sel,
+             R.data_ptr(), dev.ptr(d_ridx), frame if same else -1, out.data_ptr(),
+             ws.data_ptr(), dev.stream())
+    else:
+        call("hr_rmsd_nosuperpose", T.data_ptr(), n_frames, n_atoms, dev.ptr(d_idx), n_sel,
+             R.data_ptr(), dev.ptr(d_ridx), out.data_ptr(), ws.data_ptr(), dev.stream())
+    return _result(out, on_device)
+
+
+def _center_inplace_atom_major(xyz):
+    """Centre (F, N, 3) float32 coordinates in place and return the traces.
+
+    Mirrors ``rmsd/_rmsd.pyx:487-491``; rounding as the reference (double
+    sums, float mean and subtraction, float squares summed in double).
+    """
+    if dev.is_device_array(xyz):
+        assert xyz.dim() == 3 and xyz.shape[2] == 3
+        if not xyz.is_contiguous() or xyz.dtype != dev.torch().float32:
+            raise ValueError("xyz must be a C-contiguous float32 tensor")
+        traces = dev.empty_f32(xyz.shape[0])
+        call("hr_center_and_trace", xyz.data_ptr(), traces.data_ptr(), int(xyz.shape[0]),
+             int(xyz.shape[1]), dev.stream())
+        return dev.to_host(traces)
+    if not isinstance(xyz, np.ndarray) or xyz.dtype != np.float32 or not xyz.flags.c_contiguous:
+        raise ValueError("xyz must be a C-contiguous float32 array")
+    if not xyz.flags.writeable:
+        raise ValueError("buffer source array is read-only")
+    assert xyz.shape[2] == 3
+    X = dev.to_device_f32(xyz)
+    traces = dev.empty_f32(xyz.shape[0])
+    if xyz.shape[0]:
+        call("hr_center_and_trace", X.data_ptr(), traces.data_ptr(), int(xyz.shape[0]),
+             int(xyz.shape[1]), dev.stream())
+    xyz[...] = dev.to_host(X)
+    return dev.to_host(traces)
+
+
+def getMultipleRMSDs_atom_major(xyz1, xyz2, g1, g2, frame, parallel=True):
+    """RMSDs between xyz1[frame] and every frame of xyz2 (atom-major).
+
+    Mirrors ``rmsd/_rmsd.pyx:562-615``.  The caller's traces g1/g2 are
+    accepted for signature compatibility; centroids and traces are recomputed
+    in FP64 on the GPU.
+    """
+    assert xyz1.ndim == 3 and xyz2.ndim == 3 and xyz1.shape[2] == 3 and xyz2.shape[2] == 3
+    if not (xyz1.shape[1] == xyz2.shape[1]):
+        raise ValueError("Input arrays must have same number of atoms. "
+                         "found %d and %d." % (xyz1.shape[1], xyz2.shape[1]))
+    if frame >= xyz1.shape[0]:
+        raise ValueError("Cannot calculate RMSD of frame %d: xyz1 has "
+                         "only %d frames." % (frame, xyz1.shape[0]))
+    lib = load()
+    T = dev.to_device_f32(xyz2)
+    R = dev.to_device_f32(xyz1[frame])
+    n_frames, n_atoms = int(T.shape[0]), int(T.shape[1])
+    out = dev.empty_f32(n_frames)
+    if n_frames:
+        ws = dev.workspace(lib.hr_rmsd_workspace_bytes(n_atoms))
+        call("hr_rmsd_one_vs_all", T.data_ptr(), n_frames, n_atoms, None, n_atoms,
+             R.data_ptr(), None, -1, out.data_ptr(), ws.data_ptr(), dev.stream())
+    return _result(out, dev.is_device_array(xyz2))
+
+
+def getMultipleRMSDs_axis_major(xyz1, xyz2, g1, g2, frame, parallel=True):
+    """RMSDs between xyz1[frame] and every frame of xyz2, axis-major
+    (n_frames, 3, n_atoms).  Mirrors ``rmsd/_rmsd.pyx:502-557``."""
+    assert xyz1.ndim == 3 and xyz2.ndim == 3 and xyz1.shape[1] == 3 and xyz2.shape[1] == 3
+    if not (xyz1.shape[2] == xyz2.shape[2]):
+        raise ValueError("Input arrays must have same number of atoms. "
+                         "found %d and %d." % (xyz1.shape[2], xyz2.shape[2]))
+    if frame >= xyz1.shape[0]:
+        raise ValueError("Cannot calculate RMSD of frame %d: xyz1 has "
+                         "only %d frames." % (frame, xyz1.shape[0]))
+    lib = load()
+    T = dev.to_device_f32(xyz2)
+    R = dev.to_device_f32(xyz1[frame])
+    n_frames, n_atoms = int(T.shape[0]), int(T.shape[2])
+    out = dev.empty_f32(n_frames)
+    if n_frames:
+        ws = dev.workspace(lib.hr_rmsd_workspace_bytes(n_atoms))
+        call("hr_rmsd_one_vs_all_axis_major", T.data_ptr(), n_frames, n_atoms, n_atoms,
+             R.data_ptr(), n_atoms, -1, out.data_ptr(), ws.data_ptr(), dev.stream())
+    return _result(out, dev.is_device_array(xyz2))
+
+
+def superpose_atom_major(xyz_align_target, xyz_align_mobile, g_target, g_mobile,
+                         xyz_displace_mobile, target_frame, parallel=True):
+    """Rotate every frame of ``xyz_displace_mobile`` in place by the rotation
+    that best fits ``xyz_align_mobile[i]`` onto ``xyz_align_target[target_frame]``
+    (inputs centred by the caller).  Mirrors ``rmsd/_rmsd.pyx:620-674``."""
+    if not (xyz_align_target.shape[1] == xyz_align_mobile.shape[1]):
+        raise ValueError("Input arrays must have same number of atoms. "
+                         "found %d and %d." % (xyz_align_target.shape[1],
+                                               xyz_align_mobile.shape[1]))
+    if not xyz_align_mobile.shape[0] == xyz_displace_mobile.shape[0]:
+        raise ValueError("xyz_align_mobile and xyz_displace_mobile must contain the same "
+                         "number of frames")
+    lib = load()
+    on_device = dev.is_device_array(xyz_displace_mobile)
+    D = dev.to_device_f32(xyz_displace_mobile)
+    A = dev.to_device_f32(xyz_align_mobile)
+    R = dev.to_device_f32(xyz_align_target[target_frame])
+    n_frames, n_disp, n_align = int(D.shape[0]), int(D.shape[1]), int(A.shape[1])
+    if n_frames:
+        ws = dev.workspace(lib.hr_superpose_workspace_bytes(n_frames, n_align))
+        call("hr_superpose", D.data_ptr(), n_frames, n_disp, A.data_ptr(), n_align, None,
+             n_align, R.data_ptr(), None, 0, None, None, ws.data_ptr(), dev.stream())
+    if not on_device:
+        xyz_displace_mobile[...] = dev.to_host(D)
+    elif D.data_ptr() != xyz_displace_mobile.data_ptr():
+        xyz_displace_mobile.copy_(D)

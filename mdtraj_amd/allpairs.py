@@ -1,0 +1,56 @@
+"""All-pairs RMSD matrix.
+
+The reference has no all-pairs routine; its users loop ``md.rmsd(t, t, i)``
+(examples/clustering.ipynb:80-81, examples/centroids.ipynb:79-82), so row i
+of this matrix is one-vs-all with reference frame i.  The output is the
+condensed upper triangle (i < j, row-major), the layout
+``scipy.spatial.distance.squareform`` and ``scipy.cluster.hierarchy.linkage``
+take.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _device as dev
+from ._lib import HR_PAIRS_BF16X3, HR_PAIRS_F16X3, HR_PAIRS_FP32, call, load
+
+MODES = {"fp32": HR_PAIRS_FP32, "f16x3": HR_PAIRS_F16X3, "bf16x3": HR_PAIRS_BF16X3}
+
+
+def rmsd_matrix(traj, atom_indices=None, mode="f16x3", square=False, out=None):
+    """RMSD between every pair of frames of ``traj``.
+
+    mode: "fp32" (FP32 FMA Gram), "f16x3" (FP16 hi+lo split, three MFMA
+    products, FP32 accumulation) or "bf16x3" (the same with BF16).  Returns the
+    condensed float32 vector (a CUDA tensor when the input is on the device,
+    numpy otherwise), or the square matrix with ``square=True``.
+    """
+    if mode not in MODES:
+        raise ValueError(f"mode must be one of {sorted(MODES)}")
+    lib = load()
+    xyz = traj.xyz if hasattr(traj, "xyz") else traj
+    on_device = dev.is_device_array(xyz)
+    X = dev.to_device_f32(xyz)
+    F, N = int(X.shape[0]), int(X.shape[1])
+    if atom_indices is not None:
+        a = np.asarray(atom_indices)
+        if a.ndim != 1 or a.size == 0 or a.min() < 0 or a.max() >= N:
+            raise ValueError("atom_indices must be valid positive indices")
+    idx = dev.to_device_i32(atom_indices)
+    n_sel = N if atom_indices is None else len(atom_indices)
+    n_pairs = F * (F - 1) // 2
+    if out is None:
+        out = dev.empty_f32(max(n_pairs, 1))
+    nbytes = lib.hr_rmsd_all_pairs_workspace_bytes(F, n_sel, MODES[mode])
+    ws = dev.workspace(nbytes)
+    call("hr_rmsd_all_pairs", X.data_ptr(), F, N, dev.ptr(idx), n_sel, MODES[mode],
+         out.data_ptr(), ws.data_ptr(), nbytes, dev.stream())
+    out = out[:n_pairs]
+    if square:
+        t = dev.torch()
+        D = t.zeros((F, F), dtype=t.float32, device=out.device)
+        iu = t.triu_indices(F, F, offset=1, device=out.device)
+        D[iu[0], iu[1]] = out
+        D = D + D.T
+        return D if on_device else dev.to_host(D)
+    return out if on_device else dev.to_host(out)

@@ -1,0 +1,69 @@
+// Host-side state of the hiprmsd library: thread-local error text and cached
+// device properties.
+#include <mutex>
+#include <string>
+
+#include "common.h"
+
+namespace hr {
+
+static thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+void clear_error() { g_error.clear(); }
+
+static std::mutex g_dev_mutex;
+static DeviceInfo g_devices[64];
+
+const DeviceInfo* current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("hipGetDevice failed or device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    DeviceInfo& d = g_devices[dev];
+    if (d.device < 0) {
+        int cu = 0;
+        int lds = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess) {
+            set_error("hipDeviceGetAttribute failed on device %d", dev);
+            return nullptr;
+        }
+        d.cu_count = cu > 0 ? cu : 1;
+        d.lds_per_block = lds > 0 ? (size_t)lds : 65536;
+        d.device = dev;
+    }
+    return &d;
+}
+
+}  // namespace hr
+
+extern "C" {
+
+int hr_version(void) { return 100; }
+
+const char* hr_last_error(void) { return hr::g_error.c_str(); }
+
+int hr_init(int device) {
+    hr::clear_error();
+    HR_CHECK_HIP(hipSetDevice(device));
+    return hr::current_device() ? HR_OK : HR_ERR_HIP;
+}
+
+int hr_device_cu_count(void) {
+    const hr::DeviceInfo* d = hr::current_device();
+    return d ? d->cu_count : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// One-vs-all RMSD, centering, no-superpose RMSD and superposition kernels
+// (gfx950).  Layout in HBM: coordinates stay as the caller holds them
+// (atom-major (F, N, 3) float32 or axis-major (F, 3, rowstride)); nothing is
+// re-laid-out.  One wavefront owns one frame at a time and sweeps its selected
+// atoms with lane-strided loads; per-frame sums are accumulated in FP64 in
+// registers and reduced across the wave with cross-lane shuffles; the QCP
+// solve then runs one lane per frame (lane k finishes the k-th frame of the
+// wave's current group of 64).  The reference selection and the atom indices
+// are staged once per workgroup in LDS.
+#include "common.h"
+
+namespace hr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kUnroll = 8;
+constexpr int kDefaultWavesPerCu = 16;
+
+// One-vs-all workspace: [0, 64) reference sums (double sx, sy, sz, q);
+// [256, 256 + 12 n_sel) gathered reference, axis-major (x[], y[], z[]).
+constexpr size_t kRefOffset = 256;
+
+enum class Kind { Rmsd, NoSuperpose, Fit };
+
+// Gather the reference selection into axis-major planes and sum it in FP64.
+__global__ __launch_bounds__(kBlock) void ref_prep_kernel(const float* __restrict__ ref,
+                                                          const int* __restrict__ ref_idx,
+                                                          int n_sel, int axis_rowstride,
+                                                          float* __restrict__ ref_out,
+                                                          double* __restrict__ stats) {
+    double sx = 0.0, sy = 0.0, sz = 0.0, q = 0.0;
+    for (int j = threadIdx.x; j < n_sel; j += kBlock) {
+        float x, y, z;
+        if (axis_rowstride > 0) {
+            x = ref[j];
+            y = ref[axis_rowstride + j];
+            z = ref[2 * axis_rowstride + j];
+        } else {
+            const long long a = ref_idx ? ref_idx[j] : j;
+            x = ref[3 * a];
+            y = ref[3 * a + 1];
+            z = ref[3 * a + 2];
+        }
+        ref_out[j] = x;
+        ref_out[n_sel + j] = y;
+        ref_out[2 * n_sel + j] = z;
+        const double dx = x, dy = y, dz = z;
+        sx += dx;
+        sy += dy;
+        sz += dz;
+        q = fma(dx, dx, fma(dy, dy, fma(dz, dz, q)));
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    q = wave_sum(q);
+    __shared__ double part[kWavesPerBlock][4];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    if (lane == 0) {
+        part[w][0] = sx;
+        part[w][1] = sy;
+        part[w][2] = sz;
+        part[w][3] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int i = 0; i < kWavesPerBlock; ++i) s += part[i][threadIdx.x];
+        stats[threadIdx.x] = s;
+    }
+}
+
+struct Sums {
+    double sx, sy, sz, q;
+    double p[9];
+};
+
+template <Kind kKind>
+__device__ __forceinline__ void accumulate(Sums& s, float ax, float ay, float az, float rx,
+                                           float ry, float rz) {
+    const double x = ax, y = ay, z = az;
+    if constexpr (kKind == Kind::NoSuperpose) {
+        const double dx = x - (double)rx, dy = y - (double)ry, dz = z - (double)rz;
+        s.q = fma(dx, dx, fma(dy, dy, fma(dz, dz, s.q)));
+    } else {
+        const double bx = rx, by = ry, bz = rz;
+        s.sx += x;
+        s.sy += y;
+        s.sz += z;
+        s.q = fma(x, x, fma(y, y, fma(z, z, s.q)));
+        s.p[0] = fma(x, bx, s.p[0]);
+        s.p[1] = fma(x, by, s.p[1]);
+        s.p[2] = fma(x, bz, s.p[2]);
+        s.p[3] = fma(y, bx, s.p[3]);
+        s.p[4] = fma(y, by, s.p[4]);
+        s.p[5] = fma(y, bz, s.p[5]);
+        s.p[6] = fma(z, bx, s.p[6]);
+        s.p[7] = fma(z, by, s.p[7]);
+        s.p[8] = fma(z, bz, s.p[8]);
+    }
+}
+
+template <Kind kKind>
+__device__ __forceinline__ void wave_reduce(Sums& s) {
+    s.q = wave_sum(s.q);
+    if constexpr (kKind != Kind::NoSuperpose) {
+        s.sx = wave_sum(s.sx);
+        s.sy = wave_sum(s.sy);
+        s.sz = wave_sum(s.sz);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) s.p[i] = wave_sum(s.p[i]);
+    }
+}
+
+template <Kind kKind>
+__device__ __forceinline__ void take(Sums& dst, const Sums& src, bool pred) {
+    dst.q = pred ? src.q : dst.q;
+    if constexpr (kKind != Kind::NoSuperpose) {
+        dst.sx = pred ? src.sx : dst.sx;
+        dst.sy = pred ? src.sy : dst.sy;
+        dst.sz = pred ? src.sz : dst.sz;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dst.p[i] = pred ? src.p[i] : dst.p[i];
+    }
+}
+
+// Centred correlation matrix and traces from raw FP64 sums:
+//   M = sum a b^T - S_a S_b^T / N,  G = Q - |S|^2 / N.
+__device__ __forceinline__ void centred(const Sums& s, const double* rs, double n, double M[9],
+                                        double& ga, double& gb) {
+    const double inv = 1.0 / n;
+    const double sa[3] = {s.sx, s.sy, s.sz};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[3 * r + c] = s.p[3 * r + c] - sa[r] * rs[c] * inv;
+    ga = s.q - (sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2]) * inv;
+    gb = rs[3] - (rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2]) * inv;
+    ga = ga > 0.0 ? ga : 0.0;
+    gb = gb > 0.0 ? gb : 0.0;
+}
+
+// Per-frame kernel: one wave per frame, contiguous chunk of frames per wave.
+//   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
+//   kGather atom-major target with an index list
+//   kLds    reference selection and indices staged in LDS (else read via L2)
+template <Kind kKind, bool kAxis, bool kGather, bool kLds>
+__global__ __launch_bounds__(kBlock) void frames_kernel(
+    const float* __restrict__ xyz, long long n_frames, long long frame_stride, int rowstride,
+    const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
+    const double* __restrict__ ref_stats, long long self_frame, float* __restrict__ out,
+    float* __restrict__ rot_out, double* __restrict__ mu_out) {
+    extern __shared__ float smem[];
+    const float* rx;
+    const float* ry;
+    const float* rz;
+    const int* ix;
+    if constexpr (kLds) {
+        float* s_rx = smem;
+        float* s_ry = smem + n_sel;
+        float* s_rz = smem + 2 * n_sel;
+        int* s_idx = reinterpret_cast<int*>(smem + 3 * n_sel);
+        for (int j = threadIdx.x; j < n_sel; j += kBlock) {
+            s_rx[j] = ref_sel[j];
+            s_ry[j] = ref_sel[n_sel + j];
+            s_rz[j] = ref_sel[2 * n_sel + j];
+            if constexpr (kGather) s_idx[j] = idx[j];
+        }
+        __syncthreads();
+        rx = s_rx;
+        ry = s_ry;
+        rz = s_rz;
+        ix = s_idx;
+    } else {
+        rx = ref_sel;
+        ry = ref_sel + n_sel;
+        rz = ref_sel + 2 * n_sel;
+        ix = idx;
+    }
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (kKind != Kind::NoSuperpose) {
+        rs[0] = ref_stats[0];
+        rs[1] = ref_stats[1];
+        rs[2] = ref_stats[2];
+        rs[3] = ref_stats[3];
+    }
+    const double n = (double)n_sel;
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
+    const long long f_begin = wave * n_frames / n_waves;
+    const long long f_end = (wave + 1) * n_frames / n_waves;
+
+    for (long long base = f_begin; base < f_end; base += kWave) {
+        const int cnt = (int)min((long long)kWave, f_end - base);
+        Sums mine = {};
+        for (int k = 0; k < cnt; ++k) {
+            const float* fr = xyz + (base + k) * frame_stride;
+            Sums s = {};
+            for (int j0 = lane; j0 < n_sel; j0 += kWave * kUnroll) {
+                float ax[kUnroll], ay[kUnroll], az[kUnroll];
+                float bx[kUnroll], by[kUnroll], bz[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const int j = j0 + u * kWave;
+                    const bool ok = j < n_sel;
+                    if (ok) {
+                        if constexpr (kAxis) {
+                            ax[u] = fr[j];
+                            ay[u] = fr[rowstride + j];
+                            az[u] = fr[2 * rowstride + j];
+                        } else {
+                            const int a = kGather ? ix[j] : j;
+                            ax[u] = fr[3 * a];
+                            ay[u] = fr[3 * a + 1];
+                            az[u] = fr[3 * a + 2];
+                        }
+                        bx[u] = rx[j];
+                        by[u] = ry[j];
+                        bz[u] = rz[j];
+                    } else {
+                        ax[u] = ay[u] = az[u] = 0.0f;
+                        bx[u] = by[u] = bz[u] = 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                    accumulate<kKind>(s, ax[u], ay[u], az[u], bx[u], by[u], bz[u]);
+            }
+            wave_reduce<kKind>(s);
+            take<kKind>(mine, s, lane == k);
+        }
+        if (lane < cnt) {
+            const long long f = base + lane;
+            if constexpr (kKind == Kind::NoSuperpose) {
+                out[f] = (float)sqrt(mine.q / n);
+            } else {
+                double M[9], ga, gb;
+                centred(mine, rs, n, M, ga, gb);
+                if constexpr (kKind == Kind::Rmsd) {
+                    const double msd = qcp_msd(M, ga, gb, n);
+                    out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
+                } else {
+                    KeyMatrix k;
+                    const double lam = qcp_lambda(M, 0.5 * (ga + gb), &k);
+                    double R[9];
+                    qcp_rotation(k, lam, R);
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) rot_out[9 * f + i] = (float)R[i];
+                    mu_out[3 * f + 0] = mine.sx / n;
+                    mu_out[3 * f + 1] = mine.sy / n;
+                    mu_out[3 * f + 2] = mine.sz / n;
+                    if (out) {
+                        const double msd = (ga + gb - 2.0 * lam) / n;
+                        out[f] = (float)sqrt(msd > 0.0 ? msd : 0.0);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// In-place centering and traces, one wave per frame (two passes over the
+// frame; the second usually hits in L2).  Rounding as the reference:
+// double sums, float mean, float subtraction, float squares summed in double.
+__global__ __launch_bounds__(kBlock) void center_kernel(float* __restrict__ xyz,
+                                                        float* __restrict__ traces,
+                                                        long long n_frames, int n_atoms) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
+    for (long long f = wave; f < n_frames; f += n_waves) {
+        float* fr = xyz + f * 3LL * n_atoms;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int a = lane; a < n_atoms; a += kWave) {
+            sx += (double)fr[3 * a];
+            sy += (double)fr[3 * a + 1];
+            sz += (double)fr[3 * a + 2];
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        const float mx = (float)(sx / n_atoms), my = (float)(sy / n_atoms),
+                    mz = (float)(sz / n_atoms);
+        double tr = 0.0;
+        for (int a = lane; a < n_atoms; a += kWave) {
+            const float x = fr[3 * a] - mx, y = fr[3 * a + 1] - my, z = fr[3 * a + 2] - mz;
+            fr[3 * a] = x;
+            fr[3 * a + 1] = y;
+            fr[3 * a + 2] = z;
+            tr += (double)(x * x);
+            tr += (double)(y * y);
+            tr += (double)(z * z);
+        }
+        tr = wave_sum(tr);
+        if (lane == 0 && traces) traces[f] = (float)tr;
+    }
+}
+
+// x' = float(float(x - mu_f) R + mu_ref) over every atom of every frame,
+// rounding as Trajectory.superpose does (float64 offsets applied to float32
+// coordinates, float32 rotation, core/trajectory.py:1135-1171).
+__global__ __launch_bounds__(kBlock) void apply_kernel(float* __restrict__ xyz, long long n_frames,
+                                                       int n_atoms, const float* __restrict__ rot,
+                                                       const double* __restrict__ mu,
+                                                       const double* __restrict__ ref_stats,
+                                                       double n_ref, int translate) {
+    const double rmx = translate ? ref_stats[0] / n_ref : 0.0;
+    const double rmy = translate ? ref_stats[1] / n_ref : 0.0;
+    const double rmz = translate ? ref_stats[2] / n_ref : 0.0;
+    for (long long f = blockIdx.x; f < n_frames; f += gridDim.x) {
+        float R[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) R[i] = rot[9 * f + i];
+        const double mx = translate ? mu[3 * f] : 0.0;
+        const double my = translate ? mu[3 * f + 1] : 0.0;
+        const double mz = translate ? mu[3 * f + 2] : 0.0;
+        float* fr = xyz + f * 3LL * n_atoms;
+        for (int a = threadIdx.x; a < n_atoms; a += kBlock) {
+            const float x = translate ? (float)((double)fr[3 * a] - mx) : fr[3 * a];
+            const float y = translate ? (float)((double)fr[3 * a + 1] - my) : fr[3 * a + 1];
+            const float z = translate ? (float)((double)fr[3 * a + 2] - mz) : fr[3 * a + 2];
+            const float tx = x * R[0] + y * R[3] + z * R[6];
+            const float ty = x * R[1] + y * R[4] + z * R[7];
+            const float tz = x * R[2] + y * R[5] + z * R[8];
+            fr[3 * a] = translate ? (float)((double)tx + rmx) : tx;
+            fr[3 * a + 1] = translate ? (float)((double)ty + rmy) : ty;
+            fr[3 * a + 2] = translate ? (float)((double)tz + rmz) : tz;
+        }
+    }
+}
+
+struct Launch {
+    int blocks;
+    size_t lds;
+    bool use_lds;
+};
+
+int plan(long long n_frames, int n_sel, bool gather, Launch* l) {
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+    const size_t need = (size_t)n_sel * (3 * sizeof(float) + (gather ? sizeof(int) : 0));
+    const size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
+    l->use_lds = need <= cap;
+    l->lds = l->use_lds ? need : 0;
+    long long waves = (long long)d->cu_count * kDefaultWavesPerCu;
+    if (waves > n_frames) waves = n_frames;
+    long long blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    l->blocks = (int)(blocks < 1 ? 1 : blocks);
+    return HR_OK;
+}
+
+template <Kind kKind, bool kAxis, bool kGather>
+int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
+                  long long frame_stride, int rowstride, const int* idx, int n_sel,
+                  const float* ref_sel, const double* ref_stats, long long self_frame, float* out,
+                  float* rot_out, double* mu_out) {
+    if (l.use_lds)
+        hipLaunchKernelGGL((frames_kernel<kKind, kAxis, kGather, true>), dim3(l.blocks),
+                           dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride, rowstride, idx,
+                           n_sel, ref_sel, ref_stats, self_frame, out, rot_out, mu_out);
+    else
+        hipLaunchKernelGGL((frames_kernel<kKind, kAxis, kGather, false>), dim3(l.blocks),
+                           dim3(kBlock), 0, s, xyz, n_frames, frame_stride, rowstride, idx, n_sel,
+                           ref_sel, ref_stats, self_frame, out, rot_out, mu_out);
+    return launch_status("frames_kernel");
+}
+
+int prep_ref(hipStream_t s, const float* ref, const int* ref_idx, int n_sel, int axis_rowstride,
+             void* workspace, float** ref_sel, double** stats) {
+    char* ws = static_cast<char*>(workspace);
+    *stats = reinterpret_cast<double*>(ws);
+    *ref_sel = reinterpret_cast<float*>(ws + kRefOffset);
+    hipLaunchKernelGGL(ref_prep_kernel, dim3(1), dim3(kBlock), 0, s, ref, ref_idx, n_sel,
+                       axis_rowstride, *ref_sel, *stats);
+    return launch_status("ref_prep_kernel");
+}
+
+}  // namespace
+}  // namespace hr
+
+using namespace hr;
+
+extern "C" {
+
+size_t hr_rmsd_workspace_bytes(int32_t n_sel) {
+    return kRefOffset + (size_t)(n_sel > 0 ? n_sel : 0) * 3 * sizeof(float);
+}
+
+int hr_center_and_trace(float* xyz, float* traces, int64_t n_frames, int32_t n_atoms,
+                        void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms > 0, "invalid sizes n_frames=%lld n_atoms=%d",
+               (long long)n_frames, n_atoms);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(xyz != nullptr, "xyz is NULL");
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+    long long waves = (long long)d->cu_count * kDefaultWavesPerCu;
+    if (waves > n_frames) waves = n_frames;
+    const int blocks = (int)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    hipLaunchKernelGGL(center_kernel, dim3(blocks), dim3(kBlock), 0, (hipStream_t)stream, xyz,
+                       traces, (long long)n_frames, n_atoms);
+    return launch_status("center_kernel");
+}
+
+int hr_rmsd_prepare_reference(const float* ref_frame, const int32_t* ref_idx, int32_t n_sel,
+                              void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_sel > 0, "invalid n_sel=%d", n_sel);
+    HR_REQUIRE(ref_frame && workspace, "NULL pointer argument");
+    float* ref_sel;
+    double* stats;
+    return prep_ref((hipStream_t)stream, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel,
+                    &stats);
+}
+
+int hr_rmsd_one_vs_all_prepared(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                                const int32_t* atom_idx, int32_t n_sel, int64_t self_frame,
+                                float* out, const void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms_total > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_sel=%d", (long long)n_frames,
+               n_atoms_total, n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
+               "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(target && out && workspace, "NULL pointer argument");
+    const char* ws = static_cast<const char*>(workspace);
+    const double* stats = reinterpret_cast<const double*>(ws);
+    const float* ref_sel = reinterpret_cast<const float*>(ws + kRefOffset);
+    Launch l;
+    int rc = plan(n_frames, n_sel, atom_idx != nullptr, &l);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const long long stride = 3LL * n_atoms_total;
+    if (atom_idx)
+        return launch_frames<Kind::Rmsd, false, true>(l, s, target, n_frames, stride, 0, atom_idx,
+                                                      n_sel, ref_sel, stats, self_frame, out,
+                                                      nullptr, nullptr);
+    return launch_frames<Kind::Rmsd, false, false>(l, s, target, n_frames, stride, 0, nullptr,
+                                                   n_sel, ref_sel, stats, self_frame, out, nullptr,
+                                                   nullptr);
+}
+
+int hr_rmsd_one_vs_all(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                       const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                       const int32_t* ref_idx, int64_t self_frame, float* out, void* workspace,
+                       void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms_total > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_sel=%d", (long long)n_frames,
+               n_atoms_total, n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
+               "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(target && ref_frame && out && workspace, "NULL pointer argument");
+    int rc = hr_rmsd_prepare_reference(ref_frame, ref_idx, n_sel, workspace, stream);
+    if (rc) return rc;
+    return hr_rmsd_one_vs_all_prepared(target, n_frames, n_atoms_total, atom_idx, n_sel,
+                                       self_frame, out, workspace, stream);
+}
+
+int hr_rmsd_one_vs_all_axis_major(const float* target, int64_t n_frames, int32_t n_atoms,
+                                  int32_t rowstride, const float* ref_frame_axis,
+                                  int32_t ref_rowstride, int64_t self_frame, float* out,
+                                  void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms > 0 && rowstride >= n_atoms && ref_rowstride >= n_atoms,
+               "invalid sizes n_frames=%lld n_atoms=%d rowstride=%d ref_rowstride=%d",
+               (long long)n_frames, n_atoms, rowstride, ref_rowstride);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(target && ref_frame_axis && out && workspace, "NULL pointer argument");
+    hipStream_t s = (hipStream_t)stream;
+    float* ref_sel;
+    double* stats;
+    int rc = prep_ref(s, ref_frame_axis, nullptr, n_atoms, ref_rowstride, workspace, &ref_sel,
+                      &stats);
+    if (rc) return rc;
+    Launch l;
+    if ((rc = plan(n_frames, n_atoms, false, &l))) return rc;
+    return launch_frames<Kind::Rmsd, true, false>(l, s, target, n_frames, 3LL * rowstride,
+                                                  rowstride, nullptr, n_atoms, ref_sel, stats,
+                                                  self_frame, out, nullptr, nullptr);
+}
+
+int hr_rmsd_nosuperpose(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                        const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                        const int32_t* ref_idx, float* out, void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms_total > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_sel=%d", (long long)n_frames,
+               n_atoms_total, n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
+               "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(target && ref_frame && out && workspace, "NULL pointer argument");
+    hipStream_t s = (hipStream_t)stream;
+    float* ref_sel;
+    double* stats;
+    int rc = prep_ref(s, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel, &stats);
+    if (rc) return rc;
+    Launch l;
+    if ((rc = plan(n_frames, n_sel, atom_idx != nullptr, &l))) return rc;
+    const long long stride = 3LL * n_atoms_total;
+    if (atom_idx)
+        return launch_frames<Kind::NoSuperpose, false, true>(l, s, target, n_frames, stride, 0,
+                                                             atom_idx, n_sel, ref_sel, stats, -1,
+                                                             out, nullptr, nullptr);
+    return launch_frames<Kind::NoSuperpose, false, false>(l, s, target, n_frames, stride, 0,
+                                                          nullptr, n_sel, ref_sel, stats, -1, out,
+                                                          nullptr, nullptr);
+}
+
+size_t hr_superpose_workspace_bytes(int64_t n_frames, int32_t n_sel) {
+    const size_t nf = (size_t)(n_frames > 0 ? n_frames : 0);
+    size_t b = hr_rmsd_workspace_bytes(n_sel);
+    b = (b + 255) & ~(size_t)255;
+    b += nf * 3 * sizeof(double);  // per-frame centroids
+    b = (b + 255) & ~(size_t)255;
+    b += nf * 9 * sizeof(float);  // rotations when the caller passes none
+    return b;
+}
+
+int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* align_xyz,
+                 int32_t n_align_atoms, const int32_t* atom_idx, int32_t n_sel,
+                 const float* ref_frame, const int32_t* ref_idx, int32_t translate,
+                 float* rot_out, float* rmsd_out, void* workspace, void* stream) {
+    clear_error();
+    if (!align_xyz) n_align_atoms = n_atoms;
+    HR_REQUIRE(n_frames >= 0 && n_atoms > 0 && n_align_atoms > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_align_atoms=%d n_sel=%d",
+               (long long)n_frames, n_atoms, n_align_atoms, n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_align_atoms,
+               "n_sel (%d) must equal n_align_atoms (%d) without atom_idx", n_sel, n_align_atoms);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(xyz && ref_frame && workspace, "NULL pointer argument");
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = static_cast<char*>(workspace);
+    size_t off = (hr_rmsd_workspace_bytes(n_sel) + 255) & ~(size_t)255;
+    double* mu = reinterpret_cast<double*>(ws + off);
+    off += (size_t)n_frames * 3 * sizeof(double);
+    off = (off + 255) & ~(size_t)255;
+    float* rot = rot_out ? rot_out : reinterpret_cast<float*>(ws + off);
+    float* ref_sel;
+    double* stats;
+    int rc = prep_ref(s, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel, &stats);
+    if (rc) return rc;
+    Launch l;
+    if ((rc = plan(n_frames, n_sel, atom_idx != nullptr, &l))) return rc;
+    const float* fit = align_xyz ? align_xyz : xyz;
+    const long long stride = 3LL * n_align_atoms;
+    if (atom_idx)
+        rc = launch_frames<Kind::Fit, false, true>(l, s, fit, n_frames, stride, 0, atom_idx, n_sel,
+                                                   ref_sel, stats, -1, rmsd_out, rot, mu);
+    else
+        rc = launch_frames<Kind::Fit, false, false>(l, s, fit, n_frames, stride, 0, nullptr,
+                                                    n_sel, ref_sel, stats, -1, rmsd_out, rot, mu);
+    if (rc) return rc;
+    const DeviceInfo* d = current_device();
+    long long blocks = (long long)d->cu_count * 8;
+    if (blocks > n_frames) blocks = n_frames;
+    hipLaunchKernelGGL(apply_kernel, dim3((int)blocks), dim3(kBlock), 0, s, xyz,
+                       (long long)n_frames, n_atoms, rot, mu, stats, (double)n_sel,
+                       (int)(translate != 0));
+    return launch_status("apply_kernel");
+}
+
+}  // extern "C"

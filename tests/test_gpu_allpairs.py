@@ -1,0 +1,34 @@
+"""GPU all-pairs RMSD: rows equal one-vs-all, condensed layout, accuracy vs
+the FP64 oracle per mode (tolerances stated per mode)."""
+import numpy as np
+import pytest
+
+import mdtraj_amd as md
+from mdtraj_amd import gen
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+# FP32 accumulation of M over the atoms: error ~ eps32 |M| / (N r)
+TOL = {"fp32": 2e-4}
+
+
+@pytest.mark.parametrize("mode", ["fp32"])
+@pytest.mark.parametrize("F,N,n_sel", [(2, 5, None), (65, 30, None), (200, 100, 77), (300, 500, None)])
+def test_all_pairs_vs_fp64(gpu, mode, F, N, n_sel):
+    x = gen.walk(seed=F + N, n_frames=F, n_atoms=N)
+    idx = None if n_sel is None else gen.subset(1, N, n_sel)
+    got = md.rmsd_matrix(md.Trajectory(x), atom_indices=idx, mode=mode)
+    want = fp64.condensed(fp64.rmsd_matrix(x, atom_indices=idx))
+    assert got.shape == want.shape
+    err = np.abs(got - want).max()
+    print(f"all-pairs {mode} F={F} N={N}: max err {err:.3e} nm")
+    assert err < TOL[mode]
+
+
+def test_square_and_rows(gpu):
+    x = gen.walk(seed=9, n_frames=50, n_atoms=40)
+    D = md.rmsd_matrix(md.Trajectory(x), mode="fp32", square=True)
+    assert D.shape == (50, 50) and np.all(np.diag(D) == 0) and np.array_equal(D, D.T)
+    row = md.rmsd(md.Trajectory(x), md.Trajectory(x), 17)
+    assert np.abs(D[17] - row).max() < TOL["fp32"]
