@@ -80,7 +80,8 @@ def _validate(target_xyz, reference_xyz, frame, atom_indices, ref_atom_indices):
             raise ValueError(
                 "atom_indices and ref_atom_indices must have same number of atom indices. "
                 "found %d and %d." % (len(atom_indices), len(ref_atom_indices)))
-    if ref_atom_indices is not None and ref_atom_indices is not atom_indices:
+    if ref_atom_indices is not None:
+        # also when it defaults to atom_indices: those index the reference too
         ref_atom_indices = _check_indices(ref_atom_indices, reference_xyz.shape[1],
                                           "ref_atom_indices")
     assert (len(target_xyz.shape) == 3 and len(reference_xyz.shape) == 3

@@ -73,3 +73,11 @@ def test_no_cpu_fallback_without_gpu():
         md.rmsd(t, t, 0)
     with pytest.raises(RuntimeError, match="GPU"):
         _rmsd._center_inplace_atom_major(t.xyz.copy())
+
+
+def test_default_ref_indices_checked_against_reference():
+    # _rmsd.pyx:163-174: ref_atom_indices defaults to atom_indices and is
+    # range-checked against the reference's atoms
+    big, small = traj(N=12), traj(N=5)
+    with pytest.raises(ValueError, match="ref_atom_indices must be valid positive indices"):
+        md.rmsd(big, small, 0, atom_indices=[0, 8])
