@@ -215,7 +215,8 @@ def bench_all_pairs(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "f32", "f16x3": "f16x3", "bf16x3": "bf16x3"}[args.mode],
+        "dtype": {"fp32": "f32", "f16x3": "f16x3", "bf16x3": "bf16x3", "f16exact": "f16x4-exact-hi",
+                  "bf16exact": "bf16x4-exact-hi"}[args.mode],
         "data": "synthetic (seeded random walk generated in HBM)",
         "config": {"workload": f"all-pairs RMSD matrix, {F} frames x {N} atoms, mode {args.mode}",
                    "kernel_ms": ms},
@@ -231,7 +232,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs"])
     ap.add_argument("--frames", type=int, default=None)
-    ap.add_argument("--mode", default="f16x3", choices=["fp32", "f16x3", "bf16x3"])
+    ap.add_argument("--mode", default="f16exact",
+                    choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -143,15 +143,24 @@ int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ali
  * All-pairs RMSD matrix.  The reference has no such routine; its users loop
  * md.rmsd(traj, traj, i) over i (examples/clustering.ipynb:80-81), so row i
  * of this matrix equals one-vs-all with reference frame i.
- *   mode        HR_PAIRS_FP32  plain FP32 FMA kernel
- *               HR_PAIRS_F16X3 FP16 hi+lo split, three MFMA products, FP32 accumulate
- *               HR_PAIRS_BF16X3 BF16 hi+lo split, three MFMA products
+ *   mode        HR_PAIRS_FP32       plain FP32 FMA Gram
+ *               HR_PAIRS_F16X3      FP16 hi+lo split, three MFMA products
+ *                                   (hi.hi + hi.lo + lo.hi) in one FP32 accumulator
+ *               HR_PAIRS_BF16X3     the same with BF16
+ *               HR_PAIRS_F16_EXACT  FP16 split with a fixed-point integer hi whose
+ *                                   hi.hi products accumulate exactly in FP32, and
+ *                                   hi.lo + lo.hi + lo.lo in a second, small FP32
+ *                                   accumulator (four MFMA products; the accurate
+ *                                   mode: <= 1e-5 nm vs FP64)
+ *               HR_PAIRS_BF16_EXACT the same with BF16 lo parts
  *   out         float32 condensed upper triangle (i < j, row-major, the layout
  *               of scipy.spatial.distance.squareform), n_frames*(n_frames-1)/2
  * ------------------------------------------------------------------------- */
 #define HR_PAIRS_FP32 0
 #define HR_PAIRS_F16X3 1
 #define HR_PAIRS_BF16X3 2
+#define HR_PAIRS_F16_EXACT 3
+#define HR_PAIRS_BF16_EXACT 4
 size_t hr_rmsd_all_pairs_workspace_bytes(int64_t n_frames, int32_t n_sel, int32_t mode);
 int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                       const int32_t* atom_idx, int32_t n_sel, int32_t mode, float* out,

@@ -17,6 +17,8 @@ HR_OK = 0
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1
 HR_PAIRS_BF16X3 = 2
+HR_PAIRS_F16_EXACT = 3
+HR_PAIRS_BF16_EXACT = 4
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32

@@ -12,16 +12,21 @@ from __future__ import annotations
 import numpy as np
 
 from . import _device as dev
-from ._lib import HR_PAIRS_BF16X3, HR_PAIRS_F16X3, HR_PAIRS_FP32, call, load
+from ._lib import (HR_PAIRS_BF16_EXACT, HR_PAIRS_BF16X3, HR_PAIRS_F16_EXACT, HR_PAIRS_F16X3,
+                   HR_PAIRS_FP32, call, load)
 
-MODES = {"fp32": HR_PAIRS_FP32, "f16x3": HR_PAIRS_F16X3, "bf16x3": HR_PAIRS_BF16X3}
+MODES = {"fp32": HR_PAIRS_FP32, "f16x3": HR_PAIRS_F16X3, "bf16x3": HR_PAIRS_BF16X3,
+         "f16exact": HR_PAIRS_F16_EXACT, "bf16exact": HR_PAIRS_BF16_EXACT}
 
 
-def rmsd_matrix(traj, atom_indices=None, mode="f16x3", square=False, out=None):
+def rmsd_matrix(traj, atom_indices=None, mode="f16exact", square=False, out=None):
     """RMSD between every pair of frames of ``traj``.
 
-    mode: "fp32" (FP32 FMA Gram), "f16x3" (FP16 hi+lo split, three MFMA
-    products, FP32 accumulation) or "bf16x3" (the same with BF16).  Returns the
+    mode: "f16exact" (default; FP16 split with an exactly accumulated
+    fixed-point hi part and a small cross-term accumulator, four MFMA
+    products, <= 1e-5 nm vs FP64), "fp32" (FP32 FMA Gram), "f16x3" / "bf16x3"
+    (hi+lo split, three MFMA products in one FP32 accumulator: fastest, about
+    1e-4 nm on near-identical frames) or "bf16exact".  Returns the
     condensed float32 vector (a CUDA tensor when the input is on the device,
     numpy otherwise), or the square matrix with ``square=True``.
     """

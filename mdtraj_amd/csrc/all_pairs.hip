@@ -1,15 +1,26 @@
 // All-pairs RMSD matrix (gfx950).
 //
 // Layout in HBM (workspace): every frame's selection is centred in FP64 and
-// rounded once to float32, stored axis-major per frame block:
+// rounded once to float32, stored axis-major per frame:
 //   planes[f][axis][k]  f < F, axis < 3, k < K_pad (zero padded)
-// and its trace G_f = sum of squares of the ROUNDED centred coordinates
-// (FP64), so that M and G describe the same structure.  The correlation
-// matrix of a pair is M_ij[r][c] = sum_k P_r[i,k] P_c[j,k]: nine plane
-// products, i.e. the 3F x 3F Gram matrix of the stacked planes.  Only tiles on
-// or above the diagonal are computed; the QCP solve (FP64) runs in the tile
-// epilogue, one lane per pair, and the result is written into the condensed
-// upper triangle.
+// with its trace G_f in FP64.  The correlation matrix of a pair is
+// M_ij[r][c] = sum_k P_r[i,k] P_c[j,k]: nine plane products, i.e. the
+// 3F x 3F Gram matrix of the stacked planes.  Only tiles on or above the
+// diagonal are computed; the QCP solve (FP64) runs in the tile epilogue, one
+// lane per pair, and the result goes to the condensed upper triangle.
+//
+// Split-precision modes store, per frame f, axis a and part p (0 hi, 1 lo),
+//   split[(f * 6 + 2 a + p) * K_pad + k]
+// as x 2^s_f in FP16 or BF16 (a per-frame power of two), scale[f] = 2^-s_f,
+// and traces_split[f] = sum (hi + lo)^2 2^-2s_f in FP64 — the trace of exactly
+// the structure the matrix products see.
+//   three-product form: hi = round_half(x 2^s), lo = round_half(x 2^s - hi),
+//     M ~ hi.hi + hi.lo + lo.hi in one FP32 accumulator per entry;
+//   exact-hi form: hi = rint(x 2^s) is an integer with |hi| <= 2^b, where
+//     2 b + log2(K_pad) <= 24, so every partial sum of hi.hi is an integer
+//     below 2^24 and its FP32 accumulation is exact; lo = round_half(x 2^s - hi)
+//     and hi.lo + lo.hi + lo.lo go to a second accumulator whose magnitude is
+//     2^-b of the first, so its FP32 rounding stays far below the FP64 bar.
 #include "common.h"
 
 namespace hr {
@@ -17,11 +28,20 @@ namespace {
 
 constexpr int kPrepBlock = 256;
 
-// Gather + centre + round + trace, one wave per frame.
+enum class Split { ThreeProduct, ExactHi };
+
+template <typename Half>
+__device__ __forceinline__ float round_half(float x) {
+    return (float)(Half)x;
+}
+
+// Gather + centre + round + trace, one wave per frame; split planes too.
+template <typename Half, Split kSplit>
 __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
     const float* __restrict__ xyz, long long n_frames, int n_atoms_total,
-    const int* __restrict__ idx, int n_sel, int k_pad, float* __restrict__ planes,
-    double* __restrict__ traces) {
+    const int* __restrict__ idx, int n_sel, int k_pad, int hi_bits,
+    float* __restrict__ planes, double* __restrict__ traces, Half* __restrict__ split,
+    float* __restrict__ scale, double* __restrict__ traces_split) {
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (kPrepBlock / kWave) + threadIdx.x / kWave;
     const long long n_waves = (long long)gridDim.x * (kPrepBlock / kWave);
@@ -37,26 +57,61 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
         sx = wave_sum(sx) / n_sel;
         sy = wave_sum(sy) / n_sel;
         sz = wave_sum(sz) / n_sel;
-        float* px = planes + f * 3LL * k_pad;
-        float* py = px + k_pad;
-        float* pz = py + k_pad;
-        double g = 0.0;
-        for (int j = lane; j < k_pad; j += kWave) {
-            float x = 0.0f, y = 0.0f, z = 0.0f;
+        // centred, rounded coordinates of selected atom j (0 past the selection)
+        auto centred_xyz = [&](int j, float& x, float& y, float& z) {
+            x = y = z = 0.0f;
             if (j < n_sel) {
                 const long long a = idx ? idx[j] : j;
                 x = (float)((double)fr[3 * a] - sx);
                 y = (float)((double)fr[3 * a + 1] - sy);
                 z = (float)((double)fr[3 * a + 2] - sz);
             }
+        };
+        float* px = planes + f * 3LL * k_pad;
+        double g = 0.0;
+        float m = 0.0f;
+        for (int j = lane; j < k_pad; j += kWave) {
+            float x, y, z;
+            centred_xyz(j, x, y, z);
             px[j] = x;
-            py[j] = y;
-            pz[j] = z;
+            px[k_pad + j] = y;
+            px[2 * k_pad + j] = z;
             const double dx = x, dy = y, dz = z;
             g = fma(dx, dx, fma(dy, dy, fma(dz, dz, g)));
+            m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
         }
         g = wave_sum(g);
         if (lane == 0) traces[f] = g;
+        if (split == nullptr) continue;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+        int e = 0;
+        if (m > 0.0f) frexpf(m, &e);  // m in [2^(e-1), 2^e)
+        // three-product: max |x| 2^sh in [2^13, 2^14); exact-hi: below 2^hi_bits
+        const int sh = (kSplit == Split::ExactHi ? hi_bits : 14) - e;
+        const float up = ldexpf(1.0f, sh);
+        const double inv = ldexp(1.0, -sh);
+        double gs = 0.0;
+        Half* row = split + (f * 6) * (long long)k_pad;
+        for (int j = lane; j < k_pad; j += kWave) {
+            float c[3];
+            centred_xyz(j, c[0], c[1], c[2]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float xs = c[a] * up;  // exact: power-of-two scaling
+                const float hf = kSplit == Split::ExactHi ? rintf(xs) : round_half<Half>(xs);
+                const float lf = round_half<Half>(xs - hf);  // xs - hf is exact in float
+                row[(2 * a) * (long long)k_pad + j] = (Half)hf;
+                row[(2 * a + 1) * (long long)k_pad + j] = (Half)lf;
+                const double v = ((double)hf + (double)lf) * inv;
+                gs = fma(v, v, gs);
+            }
+        }
+        gs = wave_sum(gs);
+        if (lane == 0) {
+            scale[f] = ldexpf(1.0f, -sh);
+            traces_split[f] = gs;
+        }
     }
 }
 
@@ -170,9 +225,264 @@ __global__ __launch_bounds__(kF32Block) void pairs_fp32_kernel(
     }
 }
 
+// ------------------------------------------------------- split-MFMA kernel
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <typename Half> struct Mfma;
+template <> struct Mfma<_Float16> {
+    typedef f16x8 vec;
+    static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<__bf16> {
+    typedef bf16x8 vec;
+    static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kMTile = 64;    // frames per workgroup tile side (2 x 2 waves of 32)
+constexpr int kMBlock = 256;
+constexpr int kMK = 16;       // atoms per MFMA step (K of 32x32x16)
+constexpr int kRowBytes = 6 * kMK * 2;    // 6 planes (axis, hi/lo) x 16 halfs
+constexpr int kRowStride = kRowBytes + 16; // LDS padding against bank conflicts
+constexpr int kStageBytes = kMTile * kRowStride;
+// A and B double buffered; the epilogue reuses it for 2 x 9 x 8 x 64 floats
+constexpr size_t kMfmaLds = 4 * (size_t)kStageBytes;
+static_assert(2 * 9 * 8 * 64 * 4 <= kMfmaLds, "epilogue round must fit the staging LDS");
+
+// Stage one 16-atom chunk of the split planes of 64 frames into LDS:
+// 64 frames x 6 planes x 32 bytes = 768 16-byte pieces, 3 per thread.
+template <typename Half>
+__device__ __forceinline__ void stage_chunk(const Half* __restrict__ split, long long f0,
+                                            long long n_frames, int k_pad, int k0,
+                                            char* __restrict__ dst) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int piece = threadIdx.x + t * kMBlock;  // 0..767
+        const int half16 = piece & 1;                  // which 16 bytes of the 32-byte row
+        const int plane = (piece >> 1) % 6;
+        const int fr = piece / 12;
+        const long long f = f0 + fr;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (f < n_frames) {
+            const Half* src = split + (f * 6 + plane) * (long long)k_pad + k0 + half16 * 8;
+            v = *reinterpret_cast<const uint4*>(src);
+        }
+        *reinterpret_cast<uint4*>(dst + fr * kRowStride + plane * 32 + half16 * 16) = v;
+    }
+}
+
+// One 64 x 64 frame tile per workgroup; wave (wi, wj) owns 32 x 32 pairs.
+// Accumulators acc[3 r + c] collect P_r[i] . P_c[j] over the atoms (and, in
+// the exact-hi form, a second set for the small cross products).  32x32
+// output layout of v_mfma_f32_32x32x16: register q of lane l holds row
+// 8 (q / 4) + 4 (l / 32) + q % 4, column l % 32; operand lane l holds row (or
+// column) l % 32, k = 8 (l / 32) .. +8.
+template <typename Half, Split kSplit>
+__global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
+    const Half* __restrict__ split, const float* __restrict__ scale,
+    const double* __restrict__ traces, long long n_frames, int n_sel, int k_pad,
+    long long n_tiles, float* __restrict__ out) {
+    typedef typename Mfma<Half>::vec vec;
+    constexpr bool kExact = kSplit == Split::ExactHi;
+    constexpr int kSets = kExact ? 2 : 1;
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    // stage b of the A tile at lds + 2 b S, of the B tile at lds + (2 b + 1) S
+
+    const long long T = (n_frames + kMTile - 1) / kMTile;
+    const long long lin = xcd_remap(blockIdx.x, n_tiles);
+    long long ti, tj;
+    triangle_tile(lin, T, ti, tj);
+    const long long i0 = ti * kMTile, j0 = tj * kMTile;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    // a wave whose 32 x 32 block lies strictly below the diagonal has no work
+    const bool active = !(ti == tj && wi > wj);
+
+    f32x16 acc[kSets][9];
+#pragma unroll
+    for (int st = 0; st < kSets; ++st)
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[st][e][q] = 0.0f;
+
+    const int n_chunks = k_pad / kMK;
+    stage_chunk<Half>(split, i0, n_frames, k_pad, 0, lds);
+    stage_chunk<Half>(split, j0, n_frames, k_pad, 0, lds + kStageBytes);
+    __syncthreads();
+    const int arow = (wi * 32 + (lane & 31)) * kRowStride + (lane >> 5) * 16;
+    const int brow = (wj * 32 + (lane & 31)) * kRowStride + (lane >> 5) * 16;
+    for (int c = 0; c < n_chunks; ++c) {
+        const int cur = c & 1;
+        const char* sa = lds + (2 * cur) * kStageBytes;
+        const char* sb = lds + (2 * cur + 1) * kStageBytes;
+        if (c + 1 < n_chunks) {
+            char* na = lds + (2 * (cur ^ 1)) * kStageBytes;
+            stage_chunk<Half>(split, i0, n_frames, k_pad, (c + 1) * kMK, na);
+            stage_chunk<Half>(split, j0, n_frames, k_pad, (c + 1) * kMK, na + kStageBytes);
+        }
+        if (active) {
+            vec a[6], b[6];  // planes q = 2 axis + part
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                a[q] = *reinterpret_cast<const vec*>(sa + arow + q * 32);
+                b[q] = *reinterpret_cast<const vec*>(sb + brow + q * 32);
+            }
+            constexpr int X = kSets - 1;  // set of the cross products
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    acc[0][3 * r + cc] = Mfma<Half>::run(a[2 * r], b[2 * cc], acc[0][3 * r + cc]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    acc[X][3 * r + cc] =
+                        Mfma<Half>::run(a[2 * r], b[2 * cc + 1], acc[X][3 * r + cc]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    acc[X][3 * r + cc] =
+                        Mfma<Half>::run(a[2 * r + 1], b[2 * cc], acc[X][3 * r + cc]);
+            if constexpr (kExact) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+                        acc[X][3 * r + cc] =
+                            Mfma<Half>::run(a[2 * r + 1], b[2 * cc + 1], acc[X][3 * r + cc]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Epilogue in 8 rounds: in round (wv, h) wave wv dumps accumulator rows
+    // q = 8 h .. 8 h + 7 of every entry (and set) into LDS, then all 256
+    // threads solve those 512 pairs, two each.  One copy of the FP64 QCP code,
+    // and no accumulator is live while it runs.
+    float* ep = reinterpret_cast<float*>(lds);  // [set][entry][8 rows][64 lanes]
+    const double n = (double)n_sel;
+#pragma unroll 1
+    for (int rnd = 0; rnd < 8; ++rnd) {
+        const int wv = rnd >> 1, h = rnd & 1;
+        if (w == wv && active) {
+            if (h == 0) {
+#pragma unroll
+                for (int st = 0; st < kSets; ++st)
+#pragma unroll
+                    for (int e = 0; e < 9; ++e)
+#pragma unroll
+                        for (int qq = 0; qq < 8; ++qq)
+                            ep[((st * 9 + e) * 8 + qq) * 64 + lane] = acc[st][e][qq];
+            } else {
+#pragma unroll
+                for (int st = 0; st < kSets; ++st)
+#pragma unroll
+                    for (int e = 0; e < 9; ++e)
+#pragma unroll
+                        for (int qq = 0; qq < 8; ++qq)
+                            ep[((st * 9 + e) * 8 + qq) * 64 + lane] = acc[st][e][8 + qq];
+            }
+        }
+        __syncthreads();
+        const bool wv_active = !(ti == tj && (wv >> 1) > (wv & 1));
+        if (wv_active) {
+#pragma unroll 1
+            for (int r = 0; r < 2; ++r) {
+                const int p = threadIdx.x + kMBlock * r;  // 0..511
+                const int qq = p >> 6, ln = p & 63;
+                const int q = 8 * h + qq;
+                const long long i = i0 + (wv >> 1) * 32 + 8 * (q >> 2) + 4 * (ln >> 5) + (q & 3);
+                const long long j = j0 + (wv & 1) * 32 + (ln & 31);
+                if (i < j && j < n_frames) {
+                    const double sij = (double)(scale[i] * scale[j]);
+                    double M[9];
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) {
+                        double v = (double)ep[(e * 8 + qq) * 64 + ln];
+                        if constexpr (kExact) v += (double)ep[((9 + e) * 8 + qq) * 64 + ln];
+                        M[e] = v * sij;
+                    }
+                    const double msd = qcp_msd(M, traces[i], traces[j], n);
+                    out[condensed_index(i, j, n_frames)] = (float)sqrt(msd);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int k_pad_for(int n_sel) { return (n_sel + 63) / 64 * 64; }
+
+// Largest b with 2 b + log2(K_pad) <= 24 (exact FP32 accumulation of hi.hi),
+// capped at 7 (hi then also fits BF16's 8-bit significand exactly).
+int hi_bits_for(int k_pad) {
+    int lg = 0;
+    while ((1 << lg) < k_pad) ++lg;
+    int b = (24 - lg) / 2;
+    return b > 7 ? 7 : b;
+}
+
+// Workspace layout (every piece 256-byte aligned):
+//   traces F doubles | planes F x 3 x K floats |
+//   split F x 6 x K halfs | scale F floats | traces_split F doubles  (split modes)
+struct PairsWorkspace {
+    double* traces;
+    float* planes;
+    void* split;
+    float* scale;
+    double* traces_split;
+    size_t bytes;
+};
+
+PairsWorkspace carve(char* base, long long F, int K, bool split) {
+    PairsWorkspace w{};
+    size_t off = 0;
+    w.traces = reinterpret_cast<double*>(base + off);
+    off += align256((size_t)F * sizeof(double));
+    w.planes = reinterpret_cast<float*>(base + off);
+    off += align256((size_t)F * 3 * K * sizeof(float));
+    if (split) {
+        w.split = base + off;
+        off += align256((size_t)F * 6 * K * sizeof(uint16_t));
+        w.scale = reinterpret_cast<float*>(base + off);
+        off += align256((size_t)F * sizeof(float));
+        w.traces_split = reinterpret_cast<double*>(base + off);
+        off += align256((size_t)F * sizeof(double));
+    }
+    w.bytes = off;
+    return w;
+}
+
+template <typename Half, Split kSplit>
+int launch_split(hipStream_t s, const PairsWorkspace& w, const float* xyz, long long F,
+                 int n_atoms_total, const int* idx, int n_sel, int K, int prep_blocks,
+                 long long n_tiles, float* out) {
+    const int bits = hi_bits_for(K);
+    if (kSplit == Split::ExactHi && bits < 4) {
+        set_error("selection of %d atoms too large for exact hi.hi accumulation", n_sel);
+        return HR_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((pairs_prep_kernel<Half, kSplit>), dim3(prep_blocks), dim3(kPrepBlock), 0,
+                       s, xyz, F, n_atoms_total, idx, n_sel, K, bits, w.planes, w.traces,
+                       static_cast<Half*>(w.split), w.scale, w.traces_split);
+    int rc = launch_status("pairs_prep_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL((pairs_mfma_kernel<Half, kSplit>), dim3((unsigned)n_tiles), dim3(kMBlock),
+                       kMfmaLds, s, static_cast<const Half*>(w.split), w.scale, w.traces_split,
+                       F, n_sel, K, n_tiles, out);
+    return launch_status("pairs_mfma_kernel");
+}
 
 }  // namespace
 }  // namespace hr
@@ -183,11 +493,7 @@ extern "C" {
 
 size_t hr_rmsd_all_pairs_workspace_bytes(int64_t n_frames, int32_t n_sel, int32_t mode) {
     if (n_frames <= 0 || n_sel <= 0) return 256;
-    const size_t F = (size_t)n_frames, K = (size_t)k_pad_for(n_sel);
-    size_t b = align256(F * sizeof(double));    // traces
-    b += align256(F * 3 * K * sizeof(float));   // centred planes
-    if (mode != HR_PAIRS_FP32) b += align256(F * 3 * K * 2 * sizeof(uint16_t));  // hi/lo
-    return b;
+    return carve(nullptr, n_frames, k_pad_for(n_sel), mode != HR_PAIRS_FP32).bytes;
 }
 
 int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
@@ -199,38 +505,52 @@ int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                n_atoms_total, n_sel);
     HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
                "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
-    HR_REQUIRE(mode == HR_PAIRS_FP32 || mode == HR_PAIRS_F16X3 || mode == HR_PAIRS_BF16X3,
-               "unknown mode %d", mode);
+    HR_REQUIRE(mode >= HR_PAIRS_FP32 && mode <= HR_PAIRS_BF16_EXACT, "unknown mode %d", mode);
     if (n_frames < 2) return HR_OK;
     HR_REQUIRE(xyz && out && workspace, "NULL pointer argument");
     HR_REQUIRE(workspace_bytes >= hr_rmsd_all_pairs_workspace_bytes(n_frames, n_sel, mode),
                "workspace too small: %zu < %zu", workspace_bytes,
                hr_rmsd_all_pairs_workspace_bytes(n_frames, n_sel, mode));
-    if (mode != HR_PAIRS_FP32) {
-        set_error("split-precision MFMA mode %d not built yet", mode);
-        return HR_ERR_UNSUPPORTED;
-    }
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
     const int K = k_pad_for(n_sel);
-    char* ws = static_cast<char*>(workspace);
-    double* traces = reinterpret_cast<double*>(ws);
-    float* planes = reinterpret_cast<float*>(ws + align256((size_t)n_frames * sizeof(double)));
+    const bool split = mode != HR_PAIRS_FP32;
+    PairsWorkspace w = carve(static_cast<char*>(workspace), n_frames, K, split);
 
     long long pw = (long long)d->cu_count * 16;
     if (pw > n_frames) pw = n_frames;
     const int prep_blocks = (int)((pw + 3) / 4);
-    hipLaunchKernelGGL(pairs_prep_kernel, dim3(prep_blocks), dim3(kPrepBlock), 0, s, xyz,
-                       (long long)n_frames, n_atoms_total, atom_idx, n_sel, K, planes, traces);
-    int rc = launch_status("pairs_prep_kernel");
-    if (rc) return rc;
-
-    const long long T = (n_frames + kF32Tile - 1) / kF32Tile;
+    const int tile = split ? kMTile : kF32Tile;
+    const long long T = (n_frames + tile - 1) / tile;
     const long long n_tiles = T * (T + 1) / 2;
     HR_REQUIRE(n_tiles < (1LL << 31), "too many tiles (%lld)", n_tiles);
-    hipLaunchKernelGGL(pairs_fp32_kernel, dim3((unsigned)n_tiles), dim3(kF32Block), 0, s, planes,
-                       traces, (long long)n_frames, n_sel, K, n_tiles, out);
+    const long long F = n_frames;
+    switch (mode) {
+        case HR_PAIRS_F16X3:
+            return launch_split<_Float16, Split::ThreeProduct>(s, w, xyz, F, n_atoms_total,
+                                                               atom_idx, n_sel, K, prep_blocks,
+                                                               n_tiles, out);
+        case HR_PAIRS_BF16X3:
+            return launch_split<__bf16, Split::ThreeProduct>(s, w, xyz, F, n_atoms_total,
+                                                             atom_idx, n_sel, K, prep_blocks,
+                                                             n_tiles, out);
+        case HR_PAIRS_F16_EXACT:
+            return launch_split<_Float16, Split::ExactHi>(s, w, xyz, F, n_atoms_total, atom_idx,
+                                                          n_sel, K, prep_blocks, n_tiles, out);
+        case HR_PAIRS_BF16_EXACT:
+            return launch_split<__bf16, Split::ExactHi>(s, w, xyz, F, n_atoms_total, atom_idx,
+                                                        n_sel, K, prep_blocks, n_tiles, out);
+        default:
+            break;
+    }
+    hipLaunchKernelGGL((pairs_prep_kernel<_Float16, Split::ThreeProduct>), dim3(prep_blocks),
+                       dim3(kPrepBlock), 0, s, xyz, F, n_atoms_total, atom_idx, n_sel, K, 0,
+                       w.planes, w.traces, static_cast<_Float16*>(nullptr), nullptr, nullptr);
+    int rc = launch_status("pairs_prep_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(pairs_fp32_kernel, dim3((unsigned)n_tiles), dim3(kF32Block), 0, s,
+                       w.planes, w.traces, F, n_sel, K, n_tiles, out);
     return launch_status("pairs_fp32_kernel");
 }
 

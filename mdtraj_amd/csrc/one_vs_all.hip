@@ -142,7 +142,76 @@ __device__ __forceinline__ void centred(const Sums& s, const double* rs, double 
     gb = gb > 0.0 ? gb : 0.0;
 }
 
-// Per-frame kernel: one wave per frame, contiguous chunk of frames per wave.
+template <Kind kKind>
+struct SumsPerFrame {
+    static constexpr int value = kKind == Kind::NoSuperpose ? 1 : 13;
+};
+
+template <Kind kKind>
+__device__ __forceinline__ void store_sums(double* dst, const Sums& s) {
+    dst[0] = s.q;
+    if constexpr (kKind != Kind::NoSuperpose) {
+        dst[1] = s.sx;
+        dst[2] = s.sy;
+        dst[3] = s.sz;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) dst[4 + e] = s.p[e];
+    }
+}
+
+template <Kind kKind>
+__device__ __forceinline__ void load_sums(Sums& s, const double* src) {
+    s.q = src[0];
+    if constexpr (kKind != Kind::NoSuperpose) {
+        s.sx = src[1];
+        s.sy = src[2];
+        s.sz = src[3];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) s.p[e] = src[4 + e];
+    }
+}
+
+// LDS bytes of the per-wave result area (64 frames of sums per wave).
+template <Kind kKind>
+constexpr size_t result_lds_bytes() {
+    return (size_t)kWavesPerBlock * kWave * SumsPerFrame<kKind>::value * sizeof(double);
+}
+
+// Loads of item (frame fr, atom block b) into registers: lane l takes the
+// selected atoms b * 64 * U + l + 64 u, u < U (zeros past the selection).
+template <bool kAxis, bool kGather>
+__device__ __forceinline__ void load_item(const float* __restrict__ fr, int rowstride,
+                                          const int* __restrict__ ix, int n_sel, int lane, int b,
+                                          float (&ax)[kUnroll], float (&ay)[kUnroll],
+                                          float (&az)[kUnroll]) {
+    const int j0 = b * kWave * kUnroll + lane;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const int j = j0 + u * kWave;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (j < n_sel) {
+            if constexpr (kAxis) {
+                x = fr[j];
+                y = fr[rowstride + j];
+                z = fr[2 * rowstride + j];
+            } else {
+                const int a = kGather ? ix[j] : j;
+                x = fr[3 * a];
+                y = fr[3 * a + 1];
+                z = fr[3 * a + 2];
+            }
+        }
+        ax[u] = x;
+        ay[u] = y;
+        az[u] = z;
+    }
+}
+
+// Per-frame kernel: one wave per frame; each wave owns a contiguous chunk of
+// frames and walks (frame, atom block) items with the loads of the next item
+// in flight while the current one is accumulated.  A workgroup is launched
+// per resident slot only (grid sized by the occupancy API), so no block
+// waits for a second round.
 //   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
 //   kGather atom-major target with an index list
 //   kLds    reference selection and indices staged in LDS (else read via L2)
@@ -152,7 +221,12 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
     const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
     const double* __restrict__ ref_stats, long long self_frame, float* __restrict__ out,
     float* __restrict__ rot_out, double* __restrict__ mu_out) {
-    extern __shared__ float smem[];
+    constexpr int kNS = SumsPerFrame<kKind>::value;
+    extern __shared__ double smem_d[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    double* s_res = smem_d + wv * kWave * kNS;
+    float* smem = reinterpret_cast<float*>(smem_d + kWavesPerBlock * kWave * kNS);
     const float* rx;
     const float* ry;
     const float* rz;
@@ -188,53 +262,61 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
     }
     const double n = (double)n_sel;
 
-    const int lane = threadIdx.x & (kWave - 1);
-    const long long wave = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + wv;
     const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
     const long long f_begin = wave * n_frames / n_waves;
     const long long f_end = (wave + 1) * n_frames / n_waves;
+    const int n_blk = (n_sel + kWave * kUnroll - 1) / (kWave * kUnroll);
 
     for (long long base = f_begin; base < f_end; base += kWave) {
         const int cnt = (int)min((long long)kWave, f_end - base);
-        Sums mine = {};
-        for (int k = 0; k < cnt; ++k) {
-            const float* fr = xyz + (base + k) * frame_stride;
-            Sums s = {};
-            for (int j0 = lane; j0 < n_sel; j0 += kWave * kUnroll) {
-                float ax[kUnroll], ay[kUnroll], az[kUnroll];
-                float bx[kUnroll], by[kUnroll], bz[kUnroll];
+        const int n_items = cnt * n_blk;
+        float cx[kUnroll], cy[kUnroll], cz[kUnroll];
+        float nx[kUnroll], ny[kUnroll], nz[kUnroll];
+        load_item<kAxis, kGather>(xyz + base * frame_stride, rowstride, ix, n_sel, lane, 0, cx, cy, cz);
+        Sums s = {};
+        int k = 0, b = 0;
+        for (int it = 0; it < n_items; ++it) {
+            // prefetch the next item
+            int nk = k, nb = b + 1;
+            if (nb == n_blk) { nb = 0; ++nk; }
+            if (it + 1 < n_items)
+                load_item<kAxis, kGather>(xyz + (base + nk) * frame_stride, rowstride, ix, n_sel,
+                                          lane, nb, nx, ny, nz);
+            const int j0 = b * kWave * kUnroll + lane;
 #pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
-                    const int j = j0 + u * kWave;
-                    const bool ok = j < n_sel;
-                    if (ok) {
-                        if constexpr (kAxis) {
-                            ax[u] = fr[j];
-                            ay[u] = fr[rowstride + j];
-                            az[u] = fr[2 * rowstride + j];
-                        } else {
-                            const int a = kGather ? ix[j] : j;
-                            ax[u] = fr[3 * a];
-                            ay[u] = fr[3 * a + 1];
-                            az[u] = fr[3 * a + 2];
-                        }
-                        bx[u] = rx[j];
-                        by[u] = ry[j];
-                        bz[u] = rz[j];
-                    } else {
-                        ax[u] = ay[u] = az[u] = 0.0f;
-                        bx[u] = by[u] = bz[u] = 0.0f;
-                    }
+            for (int u = 0; u < kUnroll; ++u) {
+                const int j = j0 + u * kWave;
+                float bx = 0.0f, by = 0.0f, bz = 0.0f;
+                if (j < n_sel) {
+                    bx = rx[j];
+                    by = ry[j];
+                    bz = rz[j];
                 }
-#pragma unroll
-                for (int u = 0; u < kUnroll; ++u)
-                    accumulate<kKind>(s, ax[u], ay[u], az[u], bx[u], by[u], bz[u]);
+                accumulate<kKind>(s, cx[u], cy[u], cz[u], bx, by, bz);
             }
-            wave_reduce<kKind>(s);
-            take<kKind>(mine, s, lane == k);
+            if (b == n_blk - 1) {
+                wave_reduce<kKind>(s);
+                if (lane == 0) store_sums<kKind>(s_res + k * kNS, s);
+                s = Sums{};
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                cx[u] = nx[u];
+                cy[u] = ny[u];
+                cz[u] = nz[u];
+            }
+            k = nk;
+            b = nb;
         }
+        // lane 0's LDS stores must be visible to the other lanes of the wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < cnt) {
             const long long f = base + lane;
+            Sums mine;
+            load_sums<kKind>(mine, s_res + lane * kNS);
             if constexpr (kKind == Kind::NoSuperpose) {
                 out[f] = (float)sqrt(mine.q / n);
             } else {
@@ -244,12 +326,12 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
                     const double msd = qcp_msd(M, ga, gb, n);
                     out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
                 } else {
-                    KeyMatrix k;
-                    const double lam = qcp_lambda(M, 0.5 * (ga + gb), &k);
+                    KeyMatrix km;
+                    const double lam = qcp_lambda(M, 0.5 * (ga + gb), &km);
                     double R[9];
-                    qcp_rotation(k, lam, R);
+                    qcp_rotation(km, lam, R);
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) rot_out[9 * f + i] = (float)R[i];
+                    for (int e = 0; e < 9; ++e) rot_out[9 * f + e] = (float)R[e];
                     mu_out[3 * f + 0] = mine.sx / n;
                     mu_out[3 * f + 1] = mine.sy / n;
                     mu_out[3 * f + 2] = mine.sz / n;
@@ -260,6 +342,10 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
                 }
             }
         }
+        // the next group overwrites s_res: all lanes must have read it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -334,22 +420,37 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(float* __restrict__ xyz, 
 }
 
 struct Launch {
-    int blocks;
     size_t lds;
     bool use_lds;
 };
 
-int plan(long long n_frames, int n_sel, bool gather, Launch* l) {
+// LDS plan: the result area always, the reference selection (and indices)
+// when they fit in 64 KB together with it.
+template <Kind kKind>
+int plan(int n_sel, bool gather, Launch* l) {
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
+    const size_t res = result_lds_bytes<kKind>();
     const size_t need = (size_t)n_sel * (3 * sizeof(float) + (gather ? sizeof(int) : 0));
-    const size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
-    l->use_lds = need <= cap;
-    l->lds = l->use_lds ? need : 0;
-    long long waves = (long long)d->cu_count * kDefaultWavesPerCu;
-    if (waves > n_frames) waves = n_frames;
-    long long blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-    l->blocks = (int)(blocks < 1 ? 1 : blocks);
+    size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
+    l->use_lds = res + need <= cap;
+    l->lds = res + (l->use_lds ? need : 0);
+    return HR_OK;
+}
+
+// One workgroup per resident slot (occupancy API), capped by the work.
+template <typename Kernel>
+int resident_blocks(Kernel kernel, size_t lds, long long n_waves_wanted, int* blocks) {
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+    int per_cu = 0;
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(kernel), kBlock, lds));
+    if (per_cu < 1) per_cu = 1;
+    long long b = (long long)per_cu * d->cu_count;
+    const long long by_work = (n_waves_wanted + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (b > by_work) b = by_work;
+    *blocks = (int)(b < 1 ? 1 : b);
     return HR_OK;
 }
 
@@ -358,14 +459,20 @@ int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_
                   long long frame_stride, int rowstride, const int* idx, int n_sel,
                   const float* ref_sel, const double* ref_stats, long long self_frame, float* out,
                   float* rot_out, double* mu_out) {
-    if (l.use_lds)
-        hipLaunchKernelGGL((frames_kernel<kKind, kAxis, kGather, true>), dim3(l.blocks),
-                           dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride, rowstride, idx,
-                           n_sel, ref_sel, ref_stats, self_frame, out, rot_out, mu_out);
-    else
-        hipLaunchKernelGGL((frames_kernel<kKind, kAxis, kGather, false>), dim3(l.blocks),
-                           dim3(kBlock), 0, s, xyz, n_frames, frame_stride, rowstride, idx, n_sel,
-                           ref_sel, ref_stats, self_frame, out, rot_out, mu_out);
+    int blocks = 0, rc;
+    if (l.use_lds) {
+        auto k = frames_kernel<kKind, kAxis, kGather, true>;
+        if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
+                           rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
+                           mu_out);
+    } else {
+        auto k = frames_kernel<kKind, kAxis, kGather, false>;
+        if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
+                           rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
+                           mu_out);
+    }
     return launch_status("frames_kernel");
 }
 
@@ -433,7 +540,7 @@ int hr_rmsd_one_vs_all_prepared(const float* target, int64_t n_frames, int32_t n
     const double* stats = reinterpret_cast<const double*>(ws);
     const float* ref_sel = reinterpret_cast<const float*>(ws + kRefOffset);
     Launch l;
-    int rc = plan(n_frames, n_sel, atom_idx != nullptr, &l);
+    int rc = plan<Kind::Rmsd>(n_sel, atom_idx != nullptr, &l);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const long long stride = 3LL * n_atoms_total;
@@ -481,7 +588,7 @@ int hr_rmsd_one_vs_all_axis_major(const float* target, int64_t n_frames, int32_t
                       &stats);
     if (rc) return rc;
     Launch l;
-    if ((rc = plan(n_frames, n_atoms, false, &l))) return rc;
+    if ((rc = plan<Kind::Rmsd>(n_atoms, false, &l))) return rc;
     return launch_frames<Kind::Rmsd, true, false>(l, s, target, n_frames, 3LL * rowstride,
                                                   rowstride, nullptr, n_atoms, ref_sel, stats,
                                                   self_frame, out, nullptr, nullptr);
@@ -504,7 +611,7 @@ int hr_rmsd_nosuperpose(const float* target, int64_t n_frames, int32_t n_atoms_t
     int rc = prep_ref(s, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel, &stats);
     if (rc) return rc;
     Launch l;
-    if ((rc = plan(n_frames, n_sel, atom_idx != nullptr, &l))) return rc;
+    if ((rc = plan<Kind::NoSuperpose>(n_sel, atom_idx != nullptr, &l))) return rc;
     const long long stride = 3LL * n_atoms_total;
     if (atom_idx)
         return launch_frames<Kind::NoSuperpose, false, true>(l, s, target, n_frames, stride, 0,
@@ -550,7 +657,7 @@ int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ali
     int rc = prep_ref(s, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel, &stats);
     if (rc) return rc;
     Launch l;
-    if ((rc = plan(n_frames, n_sel, atom_idx != nullptr, &l))) return rc;
+    if ((rc = plan<Kind::Fit>(n_sel, atom_idx != nullptr, &l))) return rc;
     const float* fit = align_xyz ? align_xyz : xyz;
     const long long stride = 3LL * n_align_atoms;
     if (atom_idx)

@@ -9,11 +9,16 @@ from oracle import fp64
 
 pytestmark = pytest.mark.gpu
 
-# FP32 accumulation of M over the atoms: error ~ eps32 |M| / (N r)
-TOL = {"fp32": 2e-4}
+# Plain FP32 kernel: M accumulated in FP32 over the atoms, error ~ eps32 K |x|^2
+# / (N r); measured 3.5e-4 nm on the 300 x 500 walk (near-identical frames).
+# f16x3 / bf16x3: one FP32 accumulator of |M| ~ N sigma^2 -> ~2e-4 nm for
+# near-identical frames (CPU emulation 1.7e-4 / 2.2e-4 on a 200 x 500 walk).
+# f16exact: exact hi.hi accumulation -> emulated 7.4e-7 nm, bar 1e-5 nm.
+# bf16exact: BF16 lo parts carry 8 bits -> coordinates to ~2^-16 of max |x|.
+TOL = {"fp32": 1e-3, "f16x3": 1e-3, "bf16x3": 2e-3, "f16exact": 1e-5, "bf16exact": 2e-4}
 
 
-@pytest.mark.parametrize("mode", ["fp32"])
+@pytest.mark.parametrize("mode", ["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
 @pytest.mark.parametrize("F,N,n_sel", [(2, 5, None), (65, 30, None), (200, 100, 77), (300, 500, None)])
 def test_all_pairs_vs_fp64(gpu, mode, F, N, n_sel):
     x = gen.walk(seed=F + N, n_frames=F, n_atoms=N)

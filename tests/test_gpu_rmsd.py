@@ -111,7 +111,8 @@ def test_shapes_vs_fp64(gpu, n_frames, n_atoms, n_sel):
 def test_ref_atom_indices_mapping(gpu):
     # tests/test_rmsd.py:143-171: swapping halves with ref_atom_indices
     rng = np.random.default_rng(52)
-    normal = rng.standard_normal((6, 100, 3)).astype(np.float32)
+    base = rng.standard_normal((1, 100, 3))
+    normal = gen.random_rigid(rng, base + 0.05 * rng.standard_normal((6, 100, 3))).astype(np.float32)
     flipped = np.concatenate([normal[:, 50:], normal[:, :50]], axis=1)
     got = md.rmsd(T(normal), T(flipped), 2, atom_indices=np.arange(50),
                   ref_atom_indices=np.arange(50, 100))
