@@ -225,16 +225,57 @@ def bench_all_pairs(args):
     }
 
 
+# ------------------------------------------------------------- HBM ceiling
+def bench_hbm_ceiling(args):
+    """Plain float4 streaming read of a 2.4 GB buffer (the one-vs-all
+    trajectory's size): the read bandwidth the device delivers in practice."""
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev
+
+    nbytes = 12 * 100_000 * 2000
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    sink = torch.zeros(1, device="cuda")
+    best = None
+    for blocks in (1024, 2048, 4096, 8192, 16384):
+        def run():
+            _lib.call("hr_hbm_read_probe", x.data_ptr(), nbytes, sink.data_ptr(), blocks,
+                      dev.stream())
+        for _ in range(args.warmup):
+            run()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(args.steps):
+            run()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.steps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        log(f"stream read {blocks} blocks: {ms:.3f} ms, {gbs:.0f} GB/s")
+        if best is None or gbs > best[1]:
+            best = (blocks, gbs, ms)
+    return {"metric": "HBM streaming-read ceiling", "value": best[1], "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": best[2],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "float4 streaming read of 2.4 GB",
+                                             "blocks": best[0]},
+            "roofline": {"bound": "hbm", "achieved": best[1], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": best[1] / HBM_PEAK_GBS, "traffic": None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs"])
+    ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs", "hbm-ceiling"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
+                    help="register budget of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -247,6 +288,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     torch.cuda.set_device(0)
+    from mdtraj_amd import _lib
+
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, args.frames_waves)
+    if args.workload == "hbm-ceiling":
+        print(json.dumps(bench_hbm_ceiling(args)), flush=True)
+        return
     if args.workload == "one-vs-all":
         args.frames = args.frames or 100_000
         res = bench_one_vs_all(args)

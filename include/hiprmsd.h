@@ -48,6 +48,15 @@ int hr_init(int device);
 /* Number of compute units of the current device (after hr_init). */
 int hr_device_cu_count(void);
 
+/* Tuning knobs (no counterpart in the reference; set by the autotuner in
+ * mdtraj_amd/autotune.py).  Values are process-wide; 0 restores the default.
+ *   HR_TUNE_FRAMES_WAVES_PER_SIMD  register budget of the per-frame kernel:
+ *                                  3 (default, no spills) or 4 waves/SIMD */
+#define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
+#define HR_TUNE_COUNT 1
+int hr_set_tuning(int knob, int value);
+int hr_get_tuning(int knob);
+
 /* ---------------------------------------------------------------------------
  * Centering (replaces inplace_center_and_trace_atom_major,
  * rmsd/include/center.h:7-8; rmsd/src/center_generic.h:3-44).
@@ -165,6 +174,13 @@ size_t hr_rmsd_all_pairs_workspace_bytes(int64_t n_frames, int32_t n_sel, int32_
 int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                       const int32_t* atom_idx, int32_t n_sel, int32_t mode, float* out,
                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Diagnostics (no counterpart in the reference): read `bytes` of device
+ * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
+ * achieved rate is the HBM read ceiling the RMSD kernels are compared with.
+ * ------------------------------------------------------------------------- */
+int hr_hbm_read_probe(const void* data, size_t bytes, float* sink, int blocks, void* stream);
 
 #ifdef __cplusplus
 }

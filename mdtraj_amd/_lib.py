@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libhiprmsd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hiprmsd.h")
 
 HR_OK = 0
+HR_TUNE_FRAMES_WAVES_PER_SIMD = 0
+
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1
 HR_PAIRS_BF16X3 = 2
@@ -31,6 +33,8 @@ _SIGNATURES = {
     "hr_last_error": (ctypes.c_char_p, []),
     "hr_init": (ctypes.c_int, [ctypes.c_int]),
     "hr_device_cu_count": (ctypes.c_int, []),
+    "hr_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "hr_get_tuning": (ctypes.c_int, [ctypes.c_int]),
     "hr_center_and_trace": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp]),
     "hr_rmsd_workspace_bytes": (_sz, [_i32]),
     "hr_rmsd_one_vs_all": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
@@ -42,6 +46,7 @@ _SIGNATURES = {
     "hr_superpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "hr_rmsd_all_pairs_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "hr_rmsd_all_pairs": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "hr_hbm_read_probe": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
 }
 
 _lock = threading.Lock()

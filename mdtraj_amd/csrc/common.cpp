@@ -1,5 +1,6 @@
 // Host-side state of the hiprmsd library: thread-local error text and cached
 // device properties.
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -19,6 +20,10 @@ void set_error(const char* fmt, ...) {
 }
 
 void clear_error() { g_error.clear(); }
+
+static std::atomic<int> g_tuning[HR_TUNE_COUNT];
+
+int tuning(int knob) { return knob >= 0 && knob < HR_TUNE_COUNT ? g_tuning[knob].load() : 0; }
 
 static std::mutex g_dev_mutex;
 static DeviceInfo g_devices[64];
@@ -60,6 +65,22 @@ int hr_init(int device) {
     HR_CHECK_HIP(hipSetDevice(device));
     return hr::current_device() ? HR_OK : HR_ERR_HIP;
 }
+
+int hr_set_tuning(int knob, int value) {
+    hr::clear_error();
+    if (knob < 0 || knob >= HR_TUNE_COUNT) {
+        hr::set_error("unknown tuning knob %d", knob);
+        return HR_ERR_INVALID;
+    }
+    if (knob == HR_TUNE_FRAMES_WAVES_PER_SIMD && value != 0 && value != 3 && value != 4) {
+        hr::set_error("HR_TUNE_FRAMES_WAVES_PER_SIMD must be 0, 3 or 4 (got %d)", value);
+        return HR_ERR_INVALID;
+    }
+    hr::g_tuning[knob].store(value);
+    return HR_OK;
+}
+
+int hr_get_tuning(int knob) { return hr::tuning(knob); }
 
 int hr_device_cu_count(void) {
     const hr::DeviceInfo* d = hr::current_device();

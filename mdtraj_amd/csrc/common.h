@@ -40,6 +40,9 @@ const DeviceInfo* current_device();
         }                                \
     } while (0)
 
+// Process-wide tuning knobs (HR_TUNE_*), set by hr_set_tuning; 0 = default.
+int tuning(int knob);
+
 // Launch status of the last kernel launch.
 inline int launch_status(const char* what) {
     hipError_t e = hipGetLastError();
@@ -56,6 +59,26 @@ constexpr int kWave = 64;
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Cross-lane move of a double by a DPP pattern (both 32-bit halves).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over each row of 16 lanes, result in every lane of the row, all on the
+// VALU (no LDS traffic): quad_perm [1,0,3,2] and [2,3,0,1] (xor 1, xor 2),
+// then row_half_mirror (lane i <- 7 - i) and row_mirror (lane i <- 15 - i),
+// each of which pairs every lane with one holding the other half's sum.
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
     return v;
 }
 

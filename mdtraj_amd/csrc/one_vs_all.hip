@@ -177,19 +177,25 @@ constexpr size_t result_lds_bytes() {
     return (size_t)kWavesPerBlock * kWave * SumsPerFrame<kKind>::value * sizeof(double);
 }
 
-// Loads of item (frame fr, atom block b) into registers: lane l takes the
-// selected atoms b * 64 * U + l + 64 u, u < U (zeros past the selection).
+// Quarter-wave layout: 16 lanes own one frame, a wave works on 4 frames at
+// once (row r = lane / 16 holds frame base + r).  An item is (frame quad q,
+// atom block b): lane t of a row takes the selected atoms
+// b * 16 * kItemAtoms + t + 16 u, u < kItemAtoms.
+constexpr int kRowLanes = 16;
+constexpr int kItemAtoms = 4;   // atoms per lane per item
+constexpr int kRing = 4;        // items in the register ring (kRing - 1 in flight)
+
 template <bool kAxis, bool kGather>
-__device__ __forceinline__ void load_item(const float* __restrict__ fr, int rowstride,
-                                          const int* __restrict__ ix, int n_sel, int lane, int b,
-                                          float (&ax)[kUnroll], float (&ay)[kUnroll],
-                                          float (&az)[kUnroll]) {
-    const int j0 = b * kWave * kUnroll + lane;
+__device__ __forceinline__ void load_item(const float* __restrict__ fr, bool row_ok,
+                                          int rowstride, const int* __restrict__ ix, int n_sel,
+                                          int t, int b, float (&ax)[kItemAtoms],
+                                          float (&ay)[kItemAtoms], float (&az)[kItemAtoms]) {
+    const int j0 = b * kRowLanes * kItemAtoms + t;
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        const int j = j0 + u * kWave;
+    for (int u = 0; u < kItemAtoms; ++u) {
+        const int j = j0 + u * kRowLanes;
         float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (j < n_sel) {
+        if (row_ok && j < n_sel) {
             if constexpr (kAxis) {
                 x = fr[j];
                 y = fr[rowstride + j];
@@ -207,16 +213,30 @@ __device__ __forceinline__ void load_item(const float* __restrict__ fr, int rows
     }
 }
 
-// Per-frame kernel: one wave per frame; each wave owns a contiguous chunk of
-// frames and walks (frame, atom block) items with the loads of the next item
-// in flight while the current one is accumulated.  A workgroup is launched
-// per resident slot only (grid sized by the occupancy API), so no block
-// waits for a second round.
+template <Kind kKind>
+__device__ __forceinline__ void row_reduce(Sums& s) {
+    s.q = row16_sum(s.q);
+    if constexpr (kKind != Kind::NoSuperpose) {
+        s.sx = row16_sum(s.sx);
+        s.sy = row16_sum(s.sy);
+        s.sz = row16_sum(s.sz);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) s.p[e] = row16_sum(s.p[e]);
+    }
+}
+
+// Per-frame kernel.  Each wave owns a contiguous chunk of frames and walks
+// its items (frame quad, atom block) through a ring of kRing register
+// buffers, so the loads of the next kRing - 1 items are in flight while one
+// is accumulated in FP64.  At the end of a quad each row's sums are reduced
+// across its 16 lanes (DPP, VALU only) and parked in LDS; every 64 frames
+// the QCP solve runs one lane per frame.  The grid is one workgroup per
+// resident slot (occupancy API), so no block waits for a second round.
 //   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
 //   kGather atom-major target with an index list
 //   kLds    reference selection and indices staged in LDS (else read via L2)
-template <Kind kKind, bool kAxis, bool kGather, bool kLds>
-__global__ __launch_bounds__(kBlock) void frames_kernel(
+template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves>
+__global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
     const float* __restrict__ xyz, long long n_frames, long long frame_stride, int rowstride,
     const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
     const double* __restrict__ ref_stats, long long self_frame, float* __restrict__ out,
@@ -225,6 +245,7 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
     extern __shared__ double smem_d[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
+    const int row = lane / kRowLanes, t = lane % kRowLanes;
     double* s_res = smem_d + wv * kWave * kNS;
     float* smem = reinterpret_cast<float*>(smem_d + kWavesPerBlock * kWave * kNS);
     const float* rx;
@@ -266,50 +287,62 @@ __global__ __launch_bounds__(kBlock) void frames_kernel(
     const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
     const long long f_begin = wave * n_frames / n_waves;
     const long long f_end = (wave + 1) * n_frames / n_waves;
-    const int n_blk = (n_sel + kWave * kUnroll - 1) / (kWave * kUnroll);
+    const int n_blk = (n_sel + kRowLanes * kItemAtoms - 1) / (kRowLanes * kItemAtoms);
 
     for (long long base = f_begin; base < f_end; base += kWave) {
         const int cnt = (int)min((long long)kWave, f_end - base);
-        const int n_items = cnt * n_blk;
-        float cx[kUnroll], cy[kUnroll], cz[kUnroll];
-        float nx[kUnroll], ny[kUnroll], nz[kUnroll];
-        load_item<kAxis, kGather>(xyz + base * frame_stride, rowstride, ix, n_sel, lane, 0, cx, cy, cz);
-        Sums s = {};
-        int k = 0, b = 0;
-        for (int it = 0; it < n_items; ++it) {
-            // prefetch the next item
-            int nk = k, nb = b + 1;
-            if (nb == n_blk) { nb = 0; ++nk; }
-            if (it + 1 < n_items)
-                load_item<kAxis, kGather>(xyz + (base + nk) * frame_stride, rowstride, ix, n_sel,
-                                          lane, nb, nx, ny, nz);
-            const int j0 = b * kWave * kUnroll + lane;
+        const int n_quads = (cnt + 3) / 4;
+        const int n_items = n_quads * n_blk;
+        // item -> (quad, block); loads of `item` into ring slot `slot`
+        float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const int j = j0 + u * kWave;
-                float bx = 0.0f, by = 0.0f, bz = 0.0f;
-                if (j < n_sel) {
-                    bx = rx[j];
-                    by = ry[j];
-                    bz = rz[j];
-                }
-                accumulate<kKind>(s, cx[u], cy[u], cz[u], bx, by, bz);
+        for (int r = 0; r < kRing - 1; ++r) {
+            if (r < n_items) {
+                const int q = r / n_blk, b = r - q * n_blk;
+                const int k = 4 * q + row;
+                load_item<kAxis, kGather>(xyz + (base + k) * frame_stride, k < cnt, rowstride, ix,
+                                          n_sel, t, b, bx[r], by[r], bz[r]);
             }
-            if (b == n_blk - 1) {
-                wave_reduce<kKind>(s);
-                if (lane == 0) store_sums<kKind>(s_res + k * kNS, s);
-                s = Sums{};
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                cx[u] = nx[u];
-                cy[u] = ny[u];
-                cz[u] = nz[u];
-            }
-            k = nk;
-            b = nb;
         }
-        // lane 0's LDS stores must be visible to the other lanes of the wave
+        Sums s = {};
+        for (int it0 = 0; it0 < n_items; it0 += kRing) {
+#pragma unroll
+            for (int r = 0; r < kRing; ++r) {
+                const int it = it0 + r;
+                if (it < n_items) {
+                    // refill the slot freed one step ago with item it + kRing - 1
+                    const int nxt = it + kRing - 1;
+                    if (nxt < n_items) {
+                        const int q = nxt / n_blk, b = nxt - q * n_blk;
+                        const int k = 4 * q + row;
+                        const int slot = (r + kRing - 1) % kRing;  // static once unrolled
+                        load_item<kAxis, kGather>(xyz + (base + k) * frame_stride, k < cnt,
+                                                  rowstride, ix, n_sel, t, b, bx[slot], by[slot],
+                                                  bz[slot]);
+                    }
+                    const int q = it / n_blk, b = it - q * n_blk;
+                    const int j0 = b * kRowLanes * kItemAtoms + t;
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int j = j0 + u * kRowLanes;
+                        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+                        if (j < n_sel) {
+                            ux = rx[j];
+                            uy = ry[j];
+                            uz = rz[j];
+                        }
+                        accumulate<kKind>(s, bx[r][u], by[r][u], bz[r][u], ux, uy, uz);
+                    }
+                    if (b == n_blk - 1) {
+                        row_reduce<kKind>(s);
+                        const int k = 4 * q + row;
+                        if (t == 0 && k < cnt) store_sums<kKind>(s_res + k * kNS, s);
+                        s = Sums{};
+                    }
+                }
+            }
+        }
+        // the row leaders' LDS stores must be visible to every lane of the wave
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -454,26 +487,36 @@ int resident_blocks(Kernel kernel, size_t lds, long long n_waves_wanted, int* bl
     return HR_OK;
 }
 
+template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves>
+int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
+                    long long frame_stride, int rowstride, const int* idx, int n_sel,
+                    const float* ref_sel, const double* ref_stats, long long self_frame,
+                    float* out, float* rot_out, double* mu_out) {
+    int blocks = 0, rc;
+    auto k = frames_kernel<kKind, kAxis, kGather, kLds, kMinWaves>;
+    if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
+                       rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
+                       mu_out);
+    return launch_status("frames_kernel");
+}
+
+// Register budget of the per-frame kernel: 3 waves per SIMD (no spills) or 4
+// (more loads in flight, a few spilled registers); HR_TUNE_FRAMES_WAVES_PER_SIMD.
 template <Kind kKind, bool kAxis, bool kGather>
 int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
                   long long frame_stride, int rowstride, const int* idx, int n_sel,
                   const float* ref_sel, const double* ref_stats, long long self_frame, float* out,
                   float* rot_out, double* mu_out) {
-    int blocks = 0, rc;
-    if (l.use_lds) {
-        auto k = frames_kernel<kKind, kAxis, kGather, true>;
-        if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
-                           rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
-                           mu_out);
-    } else {
-        auto k = frames_kernel<kKind, kAxis, kGather, false>;
-        if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
-                           rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
-                           mu_out);
-    }
-    return launch_status("frames_kernel");
+    const bool four = tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4;
+#define HR_FRAMES_ARGS l, s, xyz, n_frames, frame_stride, rowstride, idx, n_sel, ref_sel, \
+                       ref_stats, self_frame, out, rot_out, mu_out
+    if (l.use_lds)
+        return four ? launch_frames_v<kKind, kAxis, kGather, true, 4>(HR_FRAMES_ARGS)
+                    : launch_frames_v<kKind, kAxis, kGather, true, 3>(HR_FRAMES_ARGS);
+    return four ? launch_frames_v<kKind, kAxis, kGather, false, 4>(HR_FRAMES_ARGS)
+                : launch_frames_v<kKind, kAxis, kGather, false, 3>(HR_FRAMES_ARGS);
+#undef HR_FRAMES_ARGS
 }
 
 int prep_ref(hipStream_t s, const float* ref, const int* ref_idx, int n_sel, int axis_rowstride,
