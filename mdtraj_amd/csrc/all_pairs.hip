@@ -9,8 +9,9 @@
 // diagonal are computed; the QCP solve (FP64) runs in the tile epilogue, one
 // lane per pair, and the result goes to the condensed upper triangle.
 //
-// Split-precision modes store, per frame f, axis a and part p (0 hi, 1 lo),
-//   split[(f * 6 + 2 a + p) * K_pad + k]
+// Split-precision modes store, per frame f, 32-atom chunk c, axis a and part
+// p (0 hi, 1 lo), chunk-major so that one (frame, chunk) is 384 contiguous bytes,
+//   split[((f * (K_pad / 32) + c) * 6 + 2 a + p) * 32 + k % 32]
 // as x 2^s_f in FP16 or BF16 (a per-frame power of two), scale[f] = 2^-s_f,
 // and traces_split[f] = sum (hi + lo)^2 2^-2s_f in FP64 — the trace of exactly
 // the structure the matrix products see.
@@ -27,6 +28,7 @@ namespace hr {
 namespace {
 
 constexpr int kPrepBlock = 256;
+constexpr int kChunk = 32;  // atoms per chunk of the split layout (K of one MFMA)
 
 enum class Split { ThreeProduct, ExactHi };
 
@@ -92,17 +94,18 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
         const float up = ldexpf(1.0f, sh);
         const double inv = ldexp(1.0, -sh);
         double gs = 0.0;
-        Half* row = split + (f * 6) * (long long)k_pad;
+        Half* row = split + f * 6LL * k_pad;  // this frame's k_pad / 32 chunks
         for (int j = lane; j < k_pad; j += kWave) {
             float c[3];
             centred_xyz(j, c[0], c[1], c[2]);
+            Half* chunk = row + (j / kChunk) * (6 * kChunk) + (j % kChunk);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const float xs = c[a] * up;  // exact: power-of-two scaling
                 const float hf = kSplit == Split::ExactHi ? rintf(xs) : round_half<Half>(xs);
                 const float lf = round_half<Half>(xs - hf);  // xs - hf is exact in float
-                row[(2 * a) * (long long)k_pad + j] = (Half)hf;
-                row[(2 * a + 1) * (long long)k_pad + j] = (Half)lf;
+                chunk[(2 * a) * kChunk] = (Half)hf;
+                chunk[(2 * a + 1) * kChunk] = (Half)lf;
                 const double v = ((double)hf + (double)lf) * inv;
                 gs = fma(v, v, gs);
             }
@@ -228,192 +231,222 @@ __global__ __launch_bounds__(kF32Block) void pairs_fp32_kernel(
 // ------------------------------------------------------- split-MFMA kernel
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <typename Half> struct Mfma;
 template <> struct Mfma<_Float16> {
     typedef f16x8 vec;
-    static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    static __device__ __forceinline__ f32x4 run(vec a, vec b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
 };
 template <> struct Mfma<__bf16> {
     typedef bf16x8 vec;
-    static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    static __device__ __forceinline__ f32x4 run(vec a, vec b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 
-constexpr int kMTile = 64;    // frames per workgroup tile side (2 x 2 waves of 32)
-constexpr int kMBlock = 256;
-constexpr int kMK = 16;       // atoms per MFMA step (K of 32x32x16)
-constexpr int kRowBytes = 6 * kMK * 2;    // 6 planes (axis, hi/lo) x 16 halfs
-constexpr int kRowStride = kRowBytes + 16; // LDS padding against bank conflicts
-constexpr int kStageBytes = kMTile * kRowStride;
-// A and B double buffered; the epilogue reuses it for 2 x 9 x 8 x 64 floats
-constexpr size_t kMfmaLds = 4 * (size_t)kStageBytes;
-static_assert(2 * 9 * 8 * 64 * 4 <= kMfmaLds, "epilogue round must fit the staging LDS");
+// Workgroup: 8 waves (2 per SIMD) on a 64 x 64 frame tile; wave w owns rows
+// 32 (w / 4) .. +32 (two 16-row MFMA blocks) and columns 16 (w % 4) .. +16.
+// Operands stream through a 3-stage LDS ring filled by direct global->LDS
+// loads (global_load_lds_dwordx4): a stage is one 32-atom chunk of both tiles,
+// each frame row 24 16-byte slots (6 planes x 4 quarters) padded to 25, which
+// makes the 16-lane operand reads bank-conflict free.
+constexpr int kMBlock = 512;
+constexpr int kMTile = 64;
+constexpr int kSlots = 25;                       // 24 data + 1 pad, 16 B each
+constexpr int kTileBytes = kMTile * kSlots * 16; // 25,600
+constexpr int kStageBytes = 2 * kTileBytes;      // A and B tiles
+constexpr int kStages = 3;
+constexpr size_t kMfmaLds = (size_t)kStages * kStageBytes;  // 153,600
+constexpr int kLoadsPerStage = 2 * kMTile * kSlots / kWave;  // 50 wave-instructions
+static_assert(kLoadsPerStage == 50, "load plan assumes 50 instructions per stage");
+constexpr int kSuper = 4;  // tiles per super-tile side (L2 reuse of A and B)
 
-// Stage one 16-atom chunk of the split planes of 64 frames into LDS:
-// 64 frames x 6 planes x 32 bytes = 768 16-byte pieces, 3 per thread.
+// Wave w issues instructions [first, first + count) of a stage: waves 0-1
+// seven, waves 2-7 six (2 x 7 + 6 x 6 = 50), all lanes active, so each wave
+// knows exactly how many loads of a stage it still has in flight.
+__device__ __forceinline__ int loads_first(int w) { return w < 2 ? 7 * w : 14 + 6 * (w - 2); }
+__device__ __forceinline__ int loads_count(int w) { return w < 2 ? 7 : 6; }
+
 template <typename Half>
-__device__ __forceinline__ void stage_chunk(const Half* __restrict__ split, long long f0,
-                                            long long n_frames, int k_pad, int k0,
-                                            char* __restrict__ dst) {
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        const int piece = threadIdx.x + t * kMBlock;  // 0..767
-        const int half16 = piece & 1;                  // which 16 bytes of the 32-byte row
-        const int plane = (piece >> 1) % 6;
-        const int fr = piece / 12;
-        const long long f = f0 + fr;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (f < n_frames) {
-            const Half* src = split + (f * 6 + plane) * (long long)k_pad + k0 + half16 * 8;
-            v = *reinterpret_cast<const uint4*>(src);
-        }
-        *reinterpret_cast<uint4*>(dst + fr * kRowStride + plane * 32 + half16 * 16) = v;
+__device__ __forceinline__ void issue_stage(const Half* __restrict__ split, long long i0,
+                                            long long j0, long long n_frames, int n_chunks,
+                                            int c, char* stage, int w, int lane) {
+    const int first = loads_first(w), count = loads_count(w);
+    for (int k = 0; k < count; ++k) {  // wave-uniform trip count
+        const int ins = first + k;
+        const int g = ins * kWave + lane;     // slot of this lane, < 3200
+        const int tile = g / (kMTile * kSlots);
+        const int sl = g - tile * (kMTile * kSlots);
+        const int fr = sl / kSlots;
+        int r = sl - fr * kSlots;
+        if (r == kSlots - 1) r = 0;           // pad slot: harmless duplicate load
+        long long f = (tile ? j0 : i0) + fr;
+        if (f >= n_frames) f = n_frames - 1;  // rows past the end: never written out
+        const Half* src = split + ((f * n_chunks + c) * 6) * kChunk + r * 8;
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void*>(src),
+            (__attribute__((address_space(3))) void*)(stage + ins * kWave * 16),
+            16, 0, 0);
     }
 }
 
-// One 64 x 64 frame tile per workgroup; wave (wi, wj) owns 32 x 32 pairs.
-// Accumulators acc[3 r + c] collect P_r[i] . P_c[j] over the atoms (and, in
-// the exact-hi form, a second set for the small cross products).  32x32
-// output layout of v_mfma_f32_32x32x16: register q of lane l holds row
-// 8 (q / 4) + 4 (l / 32) + q % 4, column l % 32; operand lane l holds row (or
-// column) l % 32, k = 8 (l / 32) .. +8.
+// Wait until this wave has at most one stage of loads outstanding (or none).
+__device__ __forceinline__ void wait_loads(int w, bool keep_one) {
+    // s_waitcnt encoding (gfx9): vmcnt[3:0] | expcnt 6:4 | lgkmcnt 11:8 | vmcnt[5:4] 15:14
+    if (!keep_one) __builtin_amdgcn_s_waitcnt(0x0F70);
+    else if (w < 2) __builtin_amdgcn_s_waitcnt(0x0F70 | 7);
+    else __builtin_amdgcn_s_waitcnt(0x0F70 | 6);
+}
+
+// Tile of block b: super-tiles of kSuper x kSuper tiles, upper triangle of
+// super-tiles row by row, tiles row-major inside; false for empty slots.
+__device__ __forceinline__ bool super_tile(long long b, long long T, long long& ti,
+                                           long long& tj) {
+    const long long TS = (T + kSuper - 1) / kSuper;
+    const long long s = b / (kSuper * kSuper);
+    const int in = (int)(b - s * (kSuper * kSuper));
+    long long si, sj;
+    triangle_tile(s, TS, si, sj);
+    ti = si * kSuper + in / kSuper;
+    tj = sj * kSuper + in % kSuper;
+    return ti <= tj && tj < T;
+}
+
+// Accumulators acc[set][3 r + c][rb] (16 x 16 blocks) collect P_r[i] . P_c[j]
+// over the atoms; the exact-hi form keeps hi.hi in set 0 and the cross
+// products in set 1.  16x16x32 layouts: operand lane l holds row (column)
+// l % 16, k = 8 (l / 16) .. +8; result register v of lane l holds row
+// 4 (l / 16) + v, column l % 16.
 template <typename Half, Split kSplit>
 __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
     const Half* __restrict__ split, const float* __restrict__ scale,
     const double* __restrict__ traces, long long n_frames, int n_sel, int k_pad,
-    long long n_tiles, float* __restrict__ out) {
+    long long n_blocks, float* __restrict__ out) {
     typedef typename Mfma<Half>::vec vec;
     constexpr bool kExact = kSplit == Split::ExactHi;
     constexpr int kSets = kExact ? 2 : 1;
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    // stage b of the A tile at lds + 2 b S, of the B tile at lds + (2 b + 1) S
 
     const long long T = (n_frames + kMTile - 1) / kMTile;
-    const long long lin = xcd_remap(blockIdx.x, n_tiles);
     long long ti, tj;
-    triangle_tile(lin, T, ti, tj);
+    if (!super_tile(xcd_remap(blockIdx.x, n_blocks), T, ti, tj)) return;  // whole block
     const long long i0 = ti * kMTile, j0 = tj * kMTile;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int wi = w >> 1, wj = w & 1;
-    // a wave whose 32 x 32 block lies strictly below the diagonal has no work
-    const bool active = !(ti == tj && wi > wj);
+    const int wi = w >> 2, wj = w & 3;
+    // on a diagonal tile, rows 32 wi.. vs columns 16 wj..: the wave has work
+    // unless its whole block lies below the diagonal
+    const bool active = !(ti == tj && 32 * wi >= 16 * wj + 16);
+    const int n_chunks = k_pad / kChunk;
 
-    f32x16 acc[kSets][9];
+    f32x4 acc[kSets][9][2];
 #pragma unroll
     for (int st = 0; st < kSets; ++st)
 #pragma unroll
         for (int e = 0; e < 9; ++e)
 #pragma unroll
-            for (int q = 0; q < 16; ++q) acc[st][e][q] = 0.0f;
+            for (int rb = 0; rb < 2; ++rb) acc[st][e][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int n_chunks = k_pad / kMK;
-    stage_chunk<Half>(split, i0, n_frames, k_pad, 0, lds);
-    stage_chunk<Half>(split, j0, n_frames, k_pad, 0, lds + kStageBytes);
-    __syncthreads();
-    const int arow = (wi * 32 + (lane & 31)) * kRowStride + (lane >> 5) * 16;
-    const int brow = (wj * 32 + (lane & 31)) * kRowStride + (lane >> 5) * 16;
+    // LDS byte offsets of this lane's operand rows (plane q adds 64 q)
+    const int quarter = lane >> 4;
+    int aoff[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
+    const int boff = kTileBytes + (16 * wj + (lane & 15)) * kSlots * 16 + quarter * 16;
+
+    issue_stage<Half>(split, i0, j0, n_frames, n_chunks, 0, lds, w, lane);
+    if (n_chunks > 1) issue_stage<Half>(split, i0, j0, n_frames, n_chunks, 1, lds + kStageBytes, w, lane);
     for (int c = 0; c < n_chunks; ++c) {
-        const int cur = c & 1;
-        const char* sa = lds + (2 * cur) * kStageBytes;
-        const char* sb = lds + (2 * cur + 1) * kStageBytes;
-        if (c + 1 < n_chunks) {
-            char* na = lds + (2 * (cur ^ 1)) * kStageBytes;
-            stage_chunk<Half>(split, i0, n_frames, k_pad, (c + 1) * kMK, na);
-            stage_chunk<Half>(split, j0, n_frames, k_pad, (c + 1) * kMK, na + kStageBytes);
-        }
-        if (active) {
-            vec a[6], b[6];  // planes q = 2 axis + part
+        wait_loads(w, c + 1 < n_chunks);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
+        __builtin_amdgcn_s_barrier();        // every wave's chunk-c loads have landed
+        if (c + 2 < n_chunks)
+            issue_stage<Half>(split, i0, j0, n_frames, n_chunks, c + 2,
+                              lds + ((c + 2) % kStages) * kStageBytes, w, lane);
+        if (!active) continue;
+        const char* st = lds + (c % kStages) * kStageBytes;
+        vec b[6];  // planes q = 2 axis + part of this wave's 16 columns
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                a[q] = *reinterpret_cast<const vec*>(sa + arow + q * 32);
-                b[q] = *reinterpret_cast<const vec*>(sb + brow + q * 32);
-            }
-            constexpr int X = kSets - 1;  // set of the cross products
+        for (int q = 0; q < 6; ++q) b[q] = *reinterpret_cast<const vec*>(st + boff + q * 64);
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r) {
+            vec a[2][2];  // [rb][part] of plane r
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                    acc[0][3 * r + cc] = Mfma<Half>::run(a[2 * r], b[2 * cc], acc[0][3 * r + cc]);
+            for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
+                for (int p = 0; p < 2; ++p)
+                    a[rb][p] = *reinterpret_cast<const vec*>(st + aoff[rb] + (2 * r + p) * 64);
+            constexpr int X = kSets - 1;
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                    acc[X][3 * r + cc] =
-                        Mfma<Half>::run(a[2 * r], b[2 * cc + 1], acc[X][3 * r + cc]);
+            for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[0][3 * r + cc][rb] = Mfma<Half>::run(a[rb][0], b[2 * cc], acc[0][3 * r + cc][rb]);
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                    acc[X][3 * r + cc] =
-                        Mfma<Half>::run(a[2 * r + 1], b[2 * cc], acc[X][3 * r + cc]);
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[X][3 * r + cc][rb] = Mfma<Half>::run(a[rb][0], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[X][3 * r + cc][rb] = Mfma<Half>::run(a[rb][1], b[2 * cc], acc[X][3 * r + cc][rb]);
             if constexpr (kExact) {
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
-                        acc[X][3 * r + cc] =
-                            Mfma<Half>::run(a[2 * r + 1], b[2 * cc + 1], acc[X][3 * r + cc]);
+                    for (int rb = 0; rb < 2; ++rb)
+                        acc[X][3 * r + cc][rb] =
+                            Mfma<Half>::run(a[rb][1], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
             }
         }
-        __syncthreads();
     }
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __syncthreads();
 
-    // Epilogue in 8 rounds: in round (wv, h) wave wv dumps accumulator rows
-    // q = 8 h .. 8 h + 7 of every entry (and set) into LDS, then all 256
-    // threads solve those 512 pairs, two each.  One copy of the FP64 QCP code,
-    // and no accumulator is live while it runs.
-    float* ep = reinterpret_cast<float*>(lds);  // [set][entry][8 rows][64 lanes]
+    // Epilogue, two rounds (rb = 0, 1): every wave parks its accumulators of
+    // one 16-row block in LDS ([wave][set][entry][v][lane] floats, at most
+    // 8 x 2 x 9 x 4 x 64 x 4 B = 147,456 B), then all 512 threads solve the
+    // 2,048 pairs of the round, four each.  One copy of the FP64 QCP code and
+    // no accumulator live while it runs.
+    float* ep = reinterpret_cast<float*>(lds);
     const double n = (double)n_sel;
-#pragma unroll 1
-    for (int rnd = 0; rnd < 8; ++rnd) {
-        const int wv = rnd >> 1, h = rnd & 1;
-        if (w == wv && active) {
-            if (h == 0) {
 #pragma unroll
-                for (int st = 0; st < kSets; ++st)
+    for (int rb = 0; rb < 2; ++rb) {
+        if (active) {
 #pragma unroll
-                    for (int e = 0; e < 9; ++e)
+            for (int st = 0; st < kSets; ++st)
 #pragma unroll
-                        for (int qq = 0; qq < 8; ++qq)
-                            ep[((st * 9 + e) * 8 + qq) * 64 + lane] = acc[st][e][qq];
-            } else {
+                for (int e = 0; e < 9; ++e)
 #pragma unroll
-                for (int st = 0; st < kSets; ++st)
-#pragma unroll
-                    for (int e = 0; e < 9; ++e)
-#pragma unroll
-                        for (int qq = 0; qq < 8; ++qq)
-                            ep[((st * 9 + e) * 8 + qq) * 64 + lane] = acc[st][e][8 + qq];
-            }
+                    for (int v = 0; v < 4; ++v)
+                        ep[(((w * kSets + st) * 9 + e) * 4 + v) * 64 + lane] = acc[st][e][rb][v];
         }
         __syncthreads();
-        const bool wv_active = !(ti == tj && (wv >> 1) > (wv & 1));
-        if (wv_active) {
 #pragma unroll 1
-            for (int r = 0; r < 2; ++r) {
-                const int p = threadIdx.x + kMBlock * r;  // 0..511
-                const int qq = p >> 6, ln = p & 63;
-                const int q = 8 * h + qq;
-                const long long i = i0 + (wv >> 1) * 32 + 8 * (q >> 2) + 4 * (ln >> 5) + (q & 3);
-                const long long j = j0 + (wv & 1) * 32 + (ln & 31);
-                if (i < j && j < n_frames) {
-                    const double sij = (double)(scale[i] * scale[j]);
-                    double M[9];
+        for (int k = 0; k < 4; ++k) {
+            const int p = threadIdx.x + kMBlock * k;  // 0..2047
+            const int pw = p >> 8, v = (p >> 6) & 3, ln = p & 63;
+            const int pwi = pw >> 2, pwj = pw & 3;
+            if (ti == tj && 32 * pwi >= 16 * pwj + 16) continue;  // inactive wave
+            const long long i = i0 + 32 * pwi + 16 * rb + 4 * (ln >> 4) + v;
+            const long long j = j0 + 16 * pwj + (ln & 15);
+            if (i < j && j < n_frames) {
+                const double sij = (double)(scale[i] * scale[j]);
+                double M[9];
 #pragma unroll
-                    for (int e = 0; e < 9; ++e) {
-                        double v = (double)ep[(e * 8 + qq) * 64 + ln];
-                        if constexpr (kExact) v += (double)ep[((9 + e) * 8 + qq) * 64 + ln];
-                        M[e] = v * sij;
-                    }
-                    const double msd = qcp_msd(M, traces[i], traces[j], n);
-                    out[condensed_index(i, j, n_frames)] = (float)sqrt(msd);
+                for (int e = 0; e < 9; ++e) {
+                    double val = (double)ep[(((pw * kSets + 0) * 9 + e) * 4 + v) * 64 + ln];
+                    if constexpr (kExact)
+                        val += (double)ep[(((pw * kSets + 1) * 9 + e) * 4 + v) * 64 + ln];
+                    M[e] = val * sij;
                 }
+                const double msd = qcp_msd(M, traces[i], traces[j], n);
+                out[condensed_index(i, j, n_frames)] = (float)sqrt(msd);
             }
         }
         __syncthreads();
@@ -422,7 +455,7 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-int k_pad_for(int n_sel) { return (n_sel + 63) / 64 * 64; }
+int k_pad_for(int n_sel) { return (n_sel + 63) / 64 * 64; }  // multiple of kChunk
 
 // Largest b with 2 b + log2(K_pad) <= 24 (exact FP32 accumulation of hi.hi),
 // capped at 7 (hi then also fits BF16's 8-bit significand exactly).
@@ -478,9 +511,24 @@ int launch_split(hipStream_t s, const PairsWorkspace& w, const float* xyz, long 
                        static_cast<Half*>(w.split), w.scale, w.traces_split);
     int rc = launch_status("pairs_prep_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL((pairs_mfma_kernel<Half, kSplit>), dim3((unsigned)n_tiles), dim3(kMBlock),
+    const long long T = (F + kMTile - 1) / kMTile;
+    const long long TS = (T + kSuper - 1) / kSuper;
+    const long long n_blocks = TS * (TS + 1) / 2 * kSuper * kSuper;
+    if (n_blocks >= (1LL << 31)) {
+        set_error("too many tiles (%lld)", n_blocks);
+        return HR_ERR_UNSUPPORTED;
+    }
+    (void)n_tiles;
+    static bool lds_attr_set = false;
+    if (!lds_attr_set) {
+        HR_CHECK_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void*>(pairs_mfma_kernel<Half, kSplit>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfmaLds));
+        lds_attr_set = true;
+    }
+    hipLaunchKernelGGL((pairs_mfma_kernel<Half, kSplit>), dim3((unsigned)n_blocks), dim3(kMBlock),
                        kMfmaLds, s, static_cast<const Half*>(w.split), w.scale, w.traces_split,
-                       F, n_sel, K, n_tiles, out);
+                       F, n_sel, K, n_blocks, out);
     return launch_status("pairs_mfma_kernel");
 }
 

@@ -14,7 +14,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kUnroll = 8;
 constexpr int kDefaultWavesPerCu = 16;
 
 // One-vs-all workspace: [0, 64) reference sums (double sx, sy, sz, q);
