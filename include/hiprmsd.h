@@ -176,6 +176,32 @@ int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Batched one-vs-all over many independent systems of uneven size (no
+ * counterpart in the reference, whose users loop md.rmsd per trajectory).
+ * A plan captures, once, a hipGraph of three kernels (ticket reset, reference
+ * prep, persistent scoring kernel whose waves pull (system, frame chunk)
+ * units, largest first, from an atomic ticket); each launch is one graph
+ * launch.  Plan creation allocates and copies the descriptor and unit tables;
+ * launches allocate nothing.  The systems' buffers must stay valid for the
+ * plan's lifetime.  Do not launch one plan on two streams at once.
+ *   frames_per_unit  frames per work unit, 1..64 (0 = 64); the autotuner in
+ *                    mdtraj_amd/autotune.py picks it per workload
+ * ------------------------------------------------------------------------- */
+typedef struct hr_system_desc {
+    const float* xyz;  /* (n_frames, n_atoms, 3) float32, device */
+    int64_t n_frames;
+    int32_t n_atoms;
+    int32_t ref_frame; /* scored against this frame of the same system (RMSD 0 there) */
+    float* out;        /* float32[n_frames], device */
+} hr_system_desc;
+typedef struct hr_batch_plan hr_batch_plan;
+int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
+                         int32_t frames_per_unit, hr_batch_plan** plan);
+int hr_batch_plan_launch(hr_batch_plan* plan, void* stream);
+int hr_batch_plan_destroy(hr_batch_plan* plan);
+int64_t hr_batch_plan_units(const hr_batch_plan* plan);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device
  * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
  * achieved rate is the HBM read ceiling the RMSD kernels are compared with.

@@ -46,11 +46,22 @@ _SIGNATURES = {
     "hr_superpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "hr_rmsd_all_pairs_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "hr_rmsd_all_pairs": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "hr_batch_plan_create": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "hr_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
+    "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),
+    "hr_batch_plan_units": (_i64, [_vp]),
     "hr_hbm_read_probe": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
 }
 
 _lock = threading.Lock()
 _lib = None
+
+
+class SystemDesc(ctypes.Structure):
+    """hr_system_desc of include/hiprmsd.h."""
+    _fields_ = [("xyz", ctypes.c_void_p), ("n_frames", ctypes.c_int64),
+                ("n_atoms", ctypes.c_int32), ("ref_frame", ctypes.c_int32),
+                ("out", ctypes.c_void_p)]
 
 
 class HipRmsdError(RuntimeError):

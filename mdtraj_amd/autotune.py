@@ -1,0 +1,97 @@
+"""Autotuner for the batched plan's tile shape.
+
+Candidates are (frames per work unit, waves per SIMD of the scoring kernel):
+small units balance uneven batches better, large units amortise the per-unit
+reference reads and QCP tail; 4 waves per SIMD keeps more loads in flight at
+the cost of a few spilled registers.  Each candidate's plan is built and
+timed on the actual batch (graph launches, HIP events); the winner is cached
+per batch signature (system count and log2 bins of frame and atom counts) in
+memory and in a JSON file (``$MDTRAJ_AMD_TUNE_CACHE``, default
+``~/.cache/mdtraj_amd/autotune.json``).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+from . import _device as dev
+
+CANDIDATES = [(fpu, w) for fpu in (16, 32, 64) for w in (3, 4)]
+_memory = {}
+
+
+def _cache_path():
+    return os.environ.get("MDTRAJ_AMD_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "mdtraj_amd",
+                                       "autotune.json"))
+
+
+def signature(xyz_list):
+    bins = {}
+    for x in xyz_list:
+        k = (int(math.log2(max(1, int(x.shape[0])))), int(math.log2(int(x.shape[1]))))
+        bins[k] = bins.get(k, 0) + 1
+    return f"batch:{len(xyz_list)}:" + ",".join(f"{a}.{b}x{c}" for (a, b), c in sorted(bins.items()))
+
+
+def _load_disk():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except Exception:
+        return {}
+
+
+def _store_disk(key, value):
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _load_disk()
+        data[key] = value
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1)
+    except OSError:
+        pass
+
+
+def time_plan(batch, reps=5):
+    """Mean milliseconds of one graph launch of `batch`'s plan."""
+    t = dev.torch()
+    for _ in range(2):
+        batch.run()
+    t.cuda.synchronize()
+    s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        batch.run()
+    e.record()
+    t.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def tune_batch(xyz_list, ref_frames=None, candidates=CANDIDATES, reps=5):
+    """Time every candidate; returns (best (fpu, waves), {candidate: ms})."""
+    from .batch import BatchRMSD
+
+    results = {}
+    for fpu, waves in candidates:
+        b = BatchRMSD(xyz_list, ref_frames, frames_per_unit=fpu, waves=waves)
+        results[(fpu, waves)] = time_plan(b, reps)
+        b.close()
+    best = min(results, key=results.get)
+    return best, results
+
+
+def tuned_shape(xyz_list, ref_frames=None):
+    key = signature(xyz_list)
+    if key in _memory:
+        return _memory[key]
+    disk = _load_disk().get(key)
+    if disk is not None:
+        _memory[key] = tuple(disk)
+        return _memory[key]
+    best, _ = tune_batch(xyz_list, ref_frames)
+    _memory[key] = best
+    _store_disk(key, list(best))
+    return best

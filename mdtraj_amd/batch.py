@@ -1,0 +1,94 @@
+"""Batched one-vs-all RMSD over many independent systems in one graph launch.
+
+The reference scores one trajectory per ``md.rmsd`` call; a batch of uneven
+systems (tens to thousands of atoms and frames each) is a loop of small calls
+there.  Here a plan (``hr_batch_plan_create``) captures a hipGraph whose
+persistent kernel drains a table of (system, frame chunk) work units, largest
+first, so a whole batch is one graph launch with no per-system launch cost.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _device as dev
+from ._lib import HR_TUNE_FRAMES_WAVES_PER_SIMD, SystemDesc, call, load
+
+
+class BatchRMSD:
+    """RMSD of every frame of each system to that system's ``ref_frames[s]``.
+
+    ``systems``: list of float32 (F_s, N_s, 3) arrays or CUDA tensors (moved
+    to the device once).  ``frames_per_unit``: work-unit size in frames
+    (1..64) and ``waves`` (3 or 4 waves per SIMD of the scoring kernel); None
+    picks the autotuned value for this batch (``mdtraj_amd.autotune``).  ``run()`` is one graph launch on torch's
+    current stream and returns the list of float32 CUDA tensors of RMSDs.
+    """
+
+    def __init__(self, systems, ref_frames=None, frames_per_unit=None, waves=None):
+        load()
+        self.xyz = [dev.to_device_f32(x) for x in systems]
+        for i, x in enumerate(self.xyz):
+            if x.dim() != 3 or x.shape[2] != 3:
+                raise ValueError(f"system {i}: xyz must be (n_frames, n_atoms, 3)")
+        n = len(self.xyz)
+        if n == 0:
+            raise ValueError("no systems")
+        self.ref_frames = [0] * n if ref_frames is None else [int(r) for r in ref_frames]
+        self.out = [dev.empty_f32(max(int(x.shape[0]), 1)) for x in self.xyz]
+        if frames_per_unit is None or waves is None:
+            from .autotune import tuned_shape
+
+            t_fpu, t_waves = tuned_shape(self.xyz, self.ref_frames)
+            frames_per_unit = t_fpu if frames_per_unit is None else frames_per_unit
+            waves = t_waves if waves is None else waves
+        self.frames_per_unit = int(frames_per_unit)
+        self.waves = int(waves)
+        self._plan = None
+        self._build()
+
+    def _descs(self):
+        arr = (SystemDesc * len(self.xyz))()
+        for i, (x, o) in enumerate(zip(self.xyz, self.out)):
+            arr[i] = SystemDesc(x.data_ptr(), int(x.shape[0]), int(x.shape[1]),
+                                self.ref_frames[i], o.data_ptr())
+        return arr
+
+    def _build(self):
+        lib = load()
+        plan = ctypes.c_void_p()
+        descs = self._descs()
+        prev = lib.hr_get_tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD)
+        call("hr_set_tuning", HR_TUNE_FRAMES_WAVES_PER_SIMD, self.waves)
+        try:
+            call("hr_batch_plan_create", ctypes.addressof(descs), len(self.xyz),
+                 self.frames_per_unit, ctypes.byref(plan))
+        finally:
+            lib.hr_set_tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD, prev)
+        self._plan = plan
+
+    @property
+    def n_units(self):
+        return int(load().hr_batch_plan_units(self._plan))
+
+    @property
+    def total_frames(self):
+        return sum(int(x.shape[0]) for x in self.xyz)
+
+    @property
+    def total_bytes(self):
+        return sum(12 * int(x.shape[0]) * int(x.shape[1]) for x in self.xyz)
+
+    def run(self):
+        call("hr_batch_plan_launch", self._plan, dev.stream())
+        return [o[: int(x.shape[0])] for o, x in zip(self.out, self.xyz)]
+
+    def close(self):
+        if self._plan is not None and self._plan.value:
+            load().hr_batch_plan_destroy(self._plan)
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,252 @@
+// Batched RMSD over many independent systems of uneven size in one graph
+// launch (gfx950).
+//
+// A plan holds, in device memory, one descriptor per system and a table of
+// work units (system, up to 64 consecutive frames) sorted by cost (frames x
+// atoms, largest first).  Its hipGraph has three nodes on one stream:
+//   1. a one-thread fill kernel that zeroes the unit ticket (no memset node),
+//   2. reference prep: one workgroup per system gathers its reference frame
+//      into axis-major planes and sums it in FP64,
+//   3. the persistent scoring kernel: one workgroup per resident slot; each
+//      wave takes the next unit from the ticket (vector atomic) until the table
+//      is exhausted, and runs the shared per-frame code on it (16 lanes per
+//      frame, FP64 sums, QCP one lane per frame).
+// Every wave leaves when the ticket passes the table's end, so the grid
+// drains on every launch.  A plan must not be launched on two streams at once
+// (the ticket is shared).
+#include <algorithm>
+#include <vector>
+
+#include "frames.h"
+
+struct hr_batch_plan {
+    int device = -1;
+    int n_systems = 0;
+    long long n_units = 0;
+    int frames_per_unit = 64;
+    void* d_systems = nullptr;
+    void* d_units = nullptr;
+    unsigned* d_ticket = nullptr;
+    char* d_ref = nullptr;
+    int blocks = 0;
+    size_t lds = 0;
+    long long total_frames = 0;
+    double total_bytes = 0.0;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace hr {
+namespace {
+
+struct DevSystem {
+    const float* xyz;
+    long long n_frames;
+    int n_atoms;
+    int ref_frame;
+    float* out;
+    float* ref_sel;     // 3 n_atoms floats, axis-major
+    double* ref_stats;  // sx, sy, sz, q
+};
+
+struct DevUnit {
+    int system;
+    int cnt;
+    long long base;
+};
+
+__global__ void ticket_reset_kernel(unsigned* __restrict__ ticket) {
+    if (threadIdx.x == 0) *ticket = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_ref_prep_kernel(const DevSystem* __restrict__ sys) {
+    const DevSystem s = sys[blockIdx.x];
+    if (s.n_frames == 0) return;  // block-uniform: no barrier is skipped by part of it
+    ref_prep_body(s.xyz + (long long)s.ref_frame * 3 * s.n_atoms, nullptr, s.n_atoms, 0,
+                  s.ref_sel, s.ref_stats);
+}
+
+template <int kMinWaves>
+__global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
+    const DevSystem* __restrict__ sys, const DevUnit* __restrict__ units, long long n_units,
+    unsigned* __restrict__ ticket) {
+    extern __shared__ double smem_d[];
+    constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    double* s_res = smem_d + wv * kWave * kNS;
+    for (;;) {
+        unsigned u = 0;
+        if (lane == 0) u = atomicAdd(ticket, 1u);
+        u = __shfl(u, 0, kWave);
+        if ((long long)u >= n_units) break;
+        const DevUnit unit = units[u];
+        const DevSystem s = sys[unit.system];
+        const int n = s.n_atoms;
+        const double rs[4] = {s.ref_stats[0], s.ref_stats[1], s.ref_stats[2], s.ref_stats[3]};
+        process_group<Kind::Rmsd, false, false>(
+            s.xyz, unit.base, unit.cnt, 3LL * n, 0, nullptr, n, s.ref_sel, s.ref_sel + n,
+            s.ref_sel + 2 * n, rs, (long long)s.ref_frame, s.out, nullptr, nullptr, s_res, lane);
+    }
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+void release(hr_batch_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->capture) (void)hipStreamDestroy(p->capture);
+    if (p->d_systems) (void)hipFree(p->d_systems);
+    if (p->d_units) (void)hipFree(p->d_units);
+    if (p->d_ticket) (void)hipFree(p->d_ticket);
+    if (p->d_ref) (void)hipFree(p->d_ref);
+    delete p;
+}
+
+template <int kMinWaves>
+int capture_plan(hr_batch_plan* p) {
+    auto k = batch_kernel<kMinWaves>;
+    const size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
+                       sizeof(double);
+    int per_cu = 0;
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k), kBlock, lds));
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+    long long blocks = (long long)(per_cu < 1 ? 1 : per_cu) * d->cu_count;
+    const long long by_work = (p->n_units + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > by_work) blocks = by_work;
+    if (blocks < 1) blocks = 1;
+    p->blocks = (int)blocks;
+    p->lds = lds;
+    hipStream_t s = p->capture;
+    HR_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
+    hipLaunchKernelGGL(batch_ref_prep_kernel, dim3(p->n_systems), dim3(kBlock), 0, s,
+                       static_cast<const DevSystem*>(p->d_systems));
+    hipLaunchKernelGGL(k, dim3(p->blocks), dim3(kBlock), lds, s,
+                       static_cast<const DevSystem*>(p->d_systems),
+                       static_cast<const DevUnit*>(p->d_units), p->n_units, p->d_ticket);
+    hipError_t launch = hipGetLastError();
+    hipError_t end = hipStreamEndCapture(s, &p->graph);
+    if (launch != hipSuccess || end != hipSuccess) {
+        set_error("batch graph capture failed: %s / %s", hipGetErrorString(launch),
+                  hipGetErrorString(end));
+        return HR_ERR_HIP;
+    }
+    HR_CHECK_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return HR_OK;
+}
+
+}  // namespace
+}  // namespace hr
+
+using namespace hr;
+
+extern "C" {
+
+int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
+                         int32_t frames_per_unit, hr_batch_plan** plan_out) {
+    clear_error();
+    HR_REQUIRE(plan_out != nullptr, "plan_out is NULL");
+    *plan_out = nullptr;
+    HR_REQUIRE(systems != nullptr && n_systems > 0, "no systems");
+    if (frames_per_unit == 0) frames_per_unit = 64;
+    HR_REQUIRE(frames_per_unit >= 1 && frames_per_unit <= kWave,
+               "frames_per_unit must be in [1, 64] (got %d)", frames_per_unit);
+    for (int i = 0; i < n_systems; ++i) {
+        const hr_system_desc& s = systems[i];
+        HR_REQUIRE(s.n_frames >= 0 && s.n_atoms > 0, "system %d: invalid sizes", i);
+        HR_REQUIRE(s.n_frames == 0 || (s.ref_frame >= 0 && s.ref_frame < s.n_frames),
+                   "system %d: ref_frame %d out of range", i, s.ref_frame);
+        HR_REQUIRE(s.n_frames == 0 || (s.xyz && s.out), "system %d: NULL pointer", i);
+    }
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+
+    hr_batch_plan* p = new hr_batch_plan;
+    p->device = d->device;
+    p->n_systems = n_systems;
+    p->frames_per_unit = frames_per_unit;
+
+    // reference workspace: per system planes (3 N floats) then 4 doubles
+    std::vector<size_t> ref_off(n_systems);
+    size_t ref_bytes = 0;
+    for (int i = 0; i < n_systems; ++i) {
+        ref_off[i] = ref_bytes;
+        ref_bytes += align256((size_t)systems[i].n_atoms * 3 * sizeof(float)) + 256;
+    }
+    // work units, largest first (LPT order keeps the tail short)
+    std::vector<DevUnit> units;
+    for (int i = 0; i < n_systems; ++i) {
+        for (long long b = 0; b < systems[i].n_frames; b += frames_per_unit) {
+            const long long c = std::min<long long>(frames_per_unit, systems[i].n_frames - b);
+            units.push_back(DevUnit{i, (int)c, b});
+        }
+        p->total_frames += systems[i].n_frames;
+        p->total_bytes += 12.0 * (double)systems[i].n_frames * systems[i].n_atoms;
+    }
+    std::stable_sort(units.begin(), units.end(), [&](const DevUnit& a, const DevUnit& b) {
+        return (long long)a.cnt * systems[a.system].n_atoms >
+               (long long)b.cnt * systems[b.system].n_atoms;
+    });
+    p->n_units = (long long)units.size();
+
+    int rc = HR_OK;
+    hipError_t e = hipSuccess;
+#define HR_PLAN_TRY(expr)                                                                     \
+    do {                                                                                      \
+        e = (expr);                                                                           \
+        if (e != hipSuccess) {                                                                \
+            set_error("%s failed: %s", #expr, hipGetErrorString(e));                           \
+            release(p);                                                                       \
+            return HR_ERR_HIP;                                                                \
+        }                                                                                     \
+    } while (0)
+    HR_PLAN_TRY(hipMalloc(&p->d_ref, ref_bytes > 0 ? ref_bytes : 256));
+    std::vector<DevSystem> ds(n_systems);
+    for (int i = 0; i < n_systems; ++i) {
+        const hr_system_desc& s = systems[i];
+        char* base = p->d_ref + ref_off[i];
+        ds[i] = DevSystem{s.xyz, s.n_frames, s.n_atoms, s.ref_frame, s.out,
+                          reinterpret_cast<float*>(base),
+                          reinterpret_cast<double*>(
+                              base + align256((size_t)s.n_atoms * 3 * sizeof(float)))};
+    }
+    HR_PLAN_TRY(hipMalloc(&p->d_systems, sizeof(DevSystem) * n_systems));
+    HR_PLAN_TRY(hipMalloc(&p->d_units, sizeof(DevUnit) * (units.empty() ? 1 : units.size())));
+    HR_PLAN_TRY(hipMalloc(&p->d_ticket, 256));
+    HR_PLAN_TRY(hipMemcpy(p->d_systems, ds.data(), sizeof(DevSystem) * n_systems,
+                          hipMemcpyHostToDevice));
+    if (!units.empty())
+        HR_PLAN_TRY(hipMemcpy(p->d_units, units.data(), sizeof(DevUnit) * units.size(),
+                              hipMemcpyHostToDevice));
+    HR_PLAN_TRY(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
+#undef HR_PLAN_TRY
+    rc = tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4 ? capture_plan<4>(p) : capture_plan<3>(p);
+    if (rc) {
+        release(p);
+        return rc;
+    }
+    *plan_out = p;
+    return HR_OK;
+}
+
+int hr_batch_plan_launch(hr_batch_plan* plan, void* stream) {
+    clear_error();
+    HR_REQUIRE(plan != nullptr && plan->exec != nullptr, "invalid plan");
+    HR_CHECK_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    return HR_OK;
+}
+
+int hr_batch_plan_destroy(hr_batch_plan* plan) {
+    clear_error();
+    release(plan);
+    return HR_OK;
+}
+
+int64_t hr_batch_plan_units(const hr_batch_plan* plan) { return plan ? plan->n_units : -1; }
+
+}  // extern "C"

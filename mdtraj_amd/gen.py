@@ -123,6 +123,41 @@ def walk_device(seed=1, n_frames=100_000, n_atoms=2000, sigma=0.005, box=5.0, ch
     return out
 
 
+def cube_device(gen_, n_frames, n_atoms, sigma=0.03, box=1.0, out=None):
+    """``small_cube`` generated in HBM with torch generator ``gen_``: a
+    reference uniform in a box, Gaussian noise per frame, a uniform random
+    rotation and a U(-2, 2) translation per frame.  float32 CUDA tensor."""
+    import torch
+
+    dev = gen_.device
+    ref = torch.rand((n_atoms, 3), generator=gen_, device=dev, dtype=torch.float64) * box
+    x = ref[None] + torch.randn((n_frames, n_atoms, 3), generator=gen_, device=dev,
+                                dtype=torch.float64) * sigma
+    u = torch.rand((3, n_frames), generator=gen_, device=dev, dtype=torch.float64)
+    s1, s2 = torch.sqrt(1 - u[0]), torch.sqrt(u[0])
+    a, b = 2 * np.pi * u[1], 2 * np.pi * u[2]
+    w, qx, qy, qz = s1 * torch.sin(a), s1 * torch.cos(a), s2 * torch.sin(b), s2 * torch.cos(b)
+    R = torch.stack([
+        torch.stack([1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - w * qz), 2 * (qx * qz + w * qy)], -1),
+        torch.stack([2 * (qx * qy + w * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - w * qx)], -1),
+        torch.stack([2 * (qx * qz - w * qy), 2 * (qy * qz + w * qx), 1 - 2 * (qx * qx + qy * qy)], -1),
+    ], -2)
+    t = torch.rand((n_frames, 1, 3), generator=gen_, device=dev, dtype=torch.float64) * 4 - 2
+    return (torch.bmm(x, R) + t).to(torch.float32)
+
+
+def uneven_systems_device(seed=3, n_systems=512, atoms=(50, 5000), frames=(10, 2000),
+                          sigma=0.03):
+    """The batched workload in HBM: sizes from ``uneven_sizes`` (log-uniform),
+    each system like ``small_cube`` with noise sigma.  List of CUDA tensors."""
+    import torch
+
+    nf, na = uneven_sizes(seed, n_systems, atoms, frames)
+    g = torch.Generator(device=torch.device("cuda", torch.cuda.current_device()))
+    g.manual_seed(seed)
+    return [cube_device(g, int(f), int(n), sigma) for f, n in zip(nf, na)]
+
+
 def subset(seed, n_atoms, n_sel):
     """Seeded sorted random atom subset (int64)."""
     rng = np.random.default_rng(seed)
