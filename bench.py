@@ -271,6 +271,80 @@ def bench_batched(args):
     }
 
 
+# ------------------------------------------------------------------ serving
+def bench_serving(args):
+    """16 device-resident pools (frames log-uniform [1k, 200k], atoms
+    log-uniform [100, 3000]); Poisson arrivals at 2,000 req/s, 1-64 query
+    frames each (noisy copies of pool frames), one-vs-pool RMSD; HBM budget
+    200 GB; continuous batching.  Latency = arrival -> rows on the host."""
+    import numpy as np
+    import torch
+
+    from mdtraj_amd import gen
+    from mdtraj_amd.serve import RMSDServer
+
+    rng = np.random.default_rng(4)
+    n_pools, rate, duration = 16, 2000.0, args.serve_seconds
+    nf = gen.log_uniform_int(rng, 1000, 200_000, n_pools)
+    na = gen.log_uniform_int(rng, 100, 3000, n_pools)
+    g = torch.Generator(device=torch.device("cuda", 0))
+    g.manual_seed(4)
+    srv = RMSDServer(hbm_budget=200e9)
+    samples = []
+    for f, n in zip(nf, na):
+        x = gen.cube_device(g, int(f), int(n), sigma=0.05, box=3.0)
+        srv.add_pool(x)
+        pick = torch.randint(0, int(f), (64,), generator=g, device="cuda")
+        q = x[pick] + 0.01 * torch.randn((64, int(n), 3), generator=g, device="cuda")
+        samples.append(q.cpu().numpy())
+        del x
+    torch.cuda.synchronize()
+    log(f"pools ready: {srv.resident / 1e9:.2f} GB resident")
+    n_req = int(rate * duration)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    pools = rng.integers(0, n_pools, n_req)
+    qs = rng.integers(1, 65, n_req)
+    srv.start()
+    # warm-up: one request per pool
+    for p in range(n_pools):
+        srv.submit(p, samples[p][:1]).wait(120)
+    srv.completed.clear()
+    srv.pairs_done = 0
+    t0 = time.perf_counter()
+    t_next = t0
+    reqs = []
+    for k in range(n_req):
+        t_next += gaps[k]
+        while True:
+            now = time.perf_counter()
+            if now >= t_next:
+                break
+            time.sleep(min(t_next - now, 0.0005))
+        reqs.append(srv.submit(int(pools[k]), samples[pools[k]][: qs[k]], t_arrival=t_next))
+    for r in reqs:
+        r.wait(600)
+    t1 = max(r.t_done for r in reqs)
+    srv.stop()
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    pairs = int(sum(int(q) * int(nf[p]) for q, p in zip(qs, pools)))
+    return {
+        "metric": "serving one-vs-pool RMSD: frame-pairs/s with p50/p99 latency",
+        "value": pairs / (t1 - t0), "unit": "frame-pairs/s", "n_gpus": 1,
+        "steps": n_req, "warmup": n_pools, "ms_per_step": (t1 - t0) * 1e3 / n_req,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16x4-exact-hi",
+        "data": "synthetic (pools generated in HBM, Poisson arrivals)",
+        "config": {"workload": f"{n_pools} pools, frames log-uniform [1k, 200k], atoms "
+                               f"log-uniform [100, 3000], Poisson {rate:.0f} req/s for "
+                               f"{duration:.1f} s, 1-64 query frames per request, HBM budget 200 GB",
+                   "requests": n_req, "batches": srv.batches,
+                   "resident_GB": srv.resident / 1e9,
+                   "latency_ms_p50": float(np.percentile(lat, 50)),
+                   "latency_ms_p99": float(np.percentile(lat, 99)),
+                   "latency_ms_mean": float(lat.mean()),
+                   "offered_req_per_s": rate, "achieved_req_per_s": n_req / (t1 - t0)},
+    }
+
+
 # ------------------------------------------------------------- HBM ceiling
 def bench_hbm_ceiling(args):
     """Plain float4 streaming read of a 2.4 GB buffer (the one-vs-all
@@ -315,11 +389,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched"])
+    ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
                     help="register budget of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
@@ -339,6 +414,9 @@ def main():
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, args.frames_waves)
     if args.workload == "hbm-ceiling":
         print(json.dumps(bench_hbm_ceiling(args)), flush=True)
+        return
+    if args.workload == "serving":
+        print(json.dumps(bench_serving(args)), flush=True)
         return
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)

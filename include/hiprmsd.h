@@ -176,6 +176,24 @@ int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Cross RMSD between two frame sets through the split-precision Gram (no
+ * counterpart in the reference; it is md.rmsd(target, queries, q) for every
+ * query frame q, in one pass over the target).  A set is first prepared into
+ * a self-contained block (centred, split, traces and scales) of
+ * hr_split_bytes(n_frames, n_sel, mode) bytes; prepared blocks can be kept
+ * resident (serving pools) and reused by any number of cross calls.
+ *   out[i * out_ld + j] = RMSD(set A frame i, set B frame j), float32
+ * Each entry depends only on its two frames, never on the other frames in
+ * the call, so results do not change with how queries are batched.
+ * ------------------------------------------------------------------------- */
+size_t hr_split_bytes(int64_t n_frames, int32_t n_sel, int32_t mode);
+int hr_split_prepare(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
+                     const int32_t* atom_idx, int32_t n_sel, int32_t mode, void* block,
+                     size_t block_bytes, void* stream);
+int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t n_b,
+                  int32_t n_sel, int32_t mode, float* out, int64_t out_ld, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Batched one-vs-all over many independent systems of uneven size (no
  * counterpart in the reference, whose users loop md.rmsd per trajectory).
  * A plan captures, once, a hipGraph of three kernels (ticket reset, reference

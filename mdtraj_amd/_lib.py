@@ -46,6 +46,9 @@ _SIGNATURES = {
     "hr_superpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "hr_rmsd_all_pairs_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "hr_rmsd_all_pairs": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "hr_split_bytes": (_sz, [_i64, _i32, _i32]),
+    "hr_split_prepare": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _sz, _vp]),
+    "hr_rmsd_cross": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _vp]),
     "hr_batch_plan_create": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "hr_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
     "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),

@@ -59,3 +59,40 @@ def rmsd_matrix(traj, atom_indices=None, mode="f16exact", square=False, out=None
         D = D + D.T
         return D if on_device else dev.to_host(D)
     return out if on_device else dev.to_host(out)
+
+
+class SplitSet:
+    """A frame set prepared for the split-precision Gram (centred, split into
+    hi/lo planes, traces and scales), resident in HBM."""
+
+    def __init__(self, traj, atom_indices=None, mode="f16exact"):
+        if mode not in MODES or mode == "fp32":
+            raise ValueError("SplitSet needs a split mode")
+        lib = load()
+        xyz = traj.xyz if hasattr(traj, "xyz") else traj
+        X = dev.to_device_f32(xyz)
+        self.n_frames, n_atoms = int(X.shape[0]), int(X.shape[1])
+        if atom_indices is not None:
+            a = np.asarray(atom_indices)
+            if a.ndim != 1 or a.size == 0 or a.min() < 0 or a.max() >= n_atoms:
+                raise ValueError("atom_indices must be valid positive indices")
+        idx = dev.to_device_i32(atom_indices)
+        self.n_sel = n_atoms if atom_indices is None else len(atom_indices)
+        self.mode = mode
+        self.nbytes = lib.hr_split_bytes(self.n_frames, self.n_sel, MODES[mode])
+        self.block = dev.workspace(self.nbytes)
+        call("hr_split_prepare", X.data_ptr(), self.n_frames, n_atoms, dev.ptr(idx), self.n_sel,
+             MODES[mode], self.block.data_ptr(), self.nbytes, dev.stream())
+
+
+def rmsd_cross(a, b, out=None):
+    """RMSD between every frame of prepared set ``a`` (rows) and of ``b``
+    (columns): float32 CUDA tensor (a.n_frames, b.n_frames)."""
+    if a.mode != b.mode or a.n_sel != b.n_sel:
+        raise ValueError("sets must share the mode and the number of selected atoms")
+    t = dev.torch()
+    if out is None:
+        out = t.empty((a.n_frames, b.n_frames), dtype=t.float32, device=a.block.device)
+    call("hr_rmsd_cross", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
+         a.n_sel, MODES[a.mode], out.data_ptr(), out.stride(0), dev.stream())
+    return out

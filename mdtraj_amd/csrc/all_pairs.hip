@@ -69,21 +69,23 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
                 z = (float)((double)fr[3 * a + 2] - sz);
             }
         };
-        float* px = planes + f * 3LL * k_pad;
+        float* px = planes ? planes + f * 3LL * k_pad : nullptr;
         double g = 0.0;
         float m = 0.0f;
         for (int j = lane; j < k_pad; j += kWave) {
             float x, y, z;
             centred_xyz(j, x, y, z);
-            px[j] = x;
-            px[k_pad + j] = y;
-            px[2 * k_pad + j] = z;
+            if (planes) {
+                px[j] = x;
+                px[k_pad + j] = y;
+                px[2 * k_pad + j] = z;
+            }
             const double dx = x, dy = y, dz = z;
             g = fma(dx, dx, fma(dy, dy, fma(dz, dz, g)));
             m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
         }
         g = wave_sum(g);
-        if (lane == 0) traces[f] = g;
+        if (lane == 0 && traces) traces[f] = g;
         if (split == nullptr) continue;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
@@ -270,26 +272,44 @@ constexpr int kSuper = 4;  // tiles per super-tile side (L2 reuse of A and B)
 __device__ __forceinline__ int loads_first(int w) { return w < 2 ? 7 * w : 14 + 6 * (w - 2); }
 __device__ __forceinline__ int loads_count(int w) { return w < 2 ? 7 : 6; }
 
+// Operand sets of one MFMA Gram launch: rows come from set A, columns from
+// set B (the same set for the all-pairs triangle).
 template <typename Half>
-__device__ __forceinline__ void issue_stage(const Half* __restrict__ split, long long i0,
-                                            long long j0, long long n_frames, int n_chunks,
-                                            int c, char* stage, int w, int lane) {
+struct GramArgs {
+    const Half* split_a;
+    const Half* split_b;
+    const float* scale_a;
+    const float* scale_b;
+    const double* traces_a;
+    const double* traces_b;
+    long long n_a, n_b;  // frames in A and B
+    int n_sel, k_pad;
+    long long n_blocks;
+    long long tiles_b;   // rectangle: tile columns
+    float* out;
+    long long out_ld;    // rectangle: row stride of out
+};
+
+template <typename Half>
+__device__ __forceinline__ void issue_stage(const GramArgs<Half>& g, long long i0, long long j0,
+                                            int n_chunks, int c, char* stage, int w, int lane) {
     const int first = loads_first(w), count = loads_count(w);
     for (int k = 0; k < count; ++k) {  // wave-uniform trip count
         const int ins = first + k;
-        const int g = ins * kWave + lane;     // slot of this lane, < 3200
-        const int tile = g / (kMTile * kSlots);
-        const int sl = g - tile * (kMTile * kSlots);
+        const int slot = ins * kWave + lane;  // < 3200
+        const int tile = slot / (kMTile * kSlots);
+        const int sl = slot - tile * (kMTile * kSlots);
         const int fr = sl / kSlots;
         int r = sl - fr * kSlots;
-        if (r == kSlots - 1) r = 0;           // pad slot: harmless duplicate load
+        if (r == kSlots - 1) r = 0;  // pad slot: harmless duplicate load
+        const long long nf = tile ? g.n_b : g.n_a;
         long long f = (tile ? j0 : i0) + fr;
-        if (f >= n_frames) f = n_frames - 1;  // rows past the end: never written out
-        const Half* src = split + ((f * n_chunks + c) * 6) * kChunk + r * 8;
+        if (f >= nf) f = nf - 1;     // rows past the end: never written out
+        const Half* base = tile ? g.split_b : g.split_a;
+        const Half* src = base + ((f * n_chunks + c) * 6) * kChunk + r * 8;
         __builtin_amdgcn_global_load_lds(
             reinterpret_cast<const void*>(src),
-            (__attribute__((address_space(3))) void*)(stage + ins * kWave * 16),
-            16, 0, 0);
+            (__attribute__((address_space(3))) void*)(stage + ins * kWave * 16), 16, 0, 0);
     }
 }
 
@@ -320,27 +340,32 @@ __device__ __forceinline__ bool super_tile(long long b, long long T, long long& 
 // products in set 1.  16x16x32 layouts: operand lane l holds row (column)
 // l % 16, k = 8 (l / 16) .. +8; result register v of lane l holds row
 // 4 (l / 16) + v, column l % 16.
-template <typename Half, Split kSplit>
-__global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
-    const Half* __restrict__ split, const float* __restrict__ scale,
-    const double* __restrict__ traces, long long n_frames, int n_sel, int k_pad,
-    long long n_blocks, float* __restrict__ out) {
+//   kRect false: upper triangle of A x A into the condensed vector;
+//   kRect true:  full A x B rectangle, out[i * out_ld + j].
+template <typename Half, Split kSplit, bool kRect>
+__global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half> g) {
     typedef typename Mfma<Half>::vec vec;
     constexpr bool kExact = kSplit == Split::ExactHi;
     constexpr int kSets = kExact ? 2 : 1;
     extern __shared__ __attribute__((aligned(16))) char lds[];
 
-    const long long T = (n_frames + kMTile - 1) / kMTile;
     long long ti, tj;
-    if (!super_tile(xcd_remap(blockIdx.x, n_blocks), T, ti, tj)) return;  // whole block
+    const long long lin = xcd_remap(blockIdx.x, g.n_blocks);
+    if constexpr (kRect) {
+        ti = lin / g.tiles_b;
+        tj = lin - ti * g.tiles_b;
+    } else {
+        const long long T = (g.n_a + kMTile - 1) / kMTile;
+        if (!super_tile(lin, T, ti, tj)) return;  // whole block
+    }
     const long long i0 = ti * kMTile, j0 = tj * kMTile;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int wi = w >> 2, wj = w & 3;
-    // on a diagonal tile, rows 32 wi.. vs columns 16 wj..: the wave has work
-    // unless its whole block lies below the diagonal
-    const bool active = !(ti == tj && 32 * wi >= 16 * wj + 16);
-    const int n_chunks = k_pad / kChunk;
+    // triangle, diagonal tile: rows 32 wi.. vs columns 16 wj..: the wave has
+    // work unless its whole block lies below the diagonal
+    const bool active = kRect || !(ti == tj && 32 * wi >= 16 * wj + 16);
+    const int n_chunks = g.k_pad / kChunk;
 
     f32x4 acc[kSets][9][2];
 #pragma unroll
@@ -354,18 +379,19 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
     const int quarter = lane >> 4;
     int aoff[2];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
+    for (int rb = 0; rb < 2; ++rb)
+        aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
     const int boff = kTileBytes + (16 * wj + (lane & 15)) * kSlots * 16 + quarter * 16;
 
-    issue_stage<Half>(split, i0, j0, n_frames, n_chunks, 0, lds, w, lane);
-    if (n_chunks > 1) issue_stage<Half>(split, i0, j0, n_frames, n_chunks, 1, lds + kStageBytes, w, lane);
+    issue_stage<Half>(g, i0, j0, n_chunks, 0, lds, w, lane);
+    if (n_chunks > 1) issue_stage<Half>(g, i0, j0, n_chunks, 1, lds + kStageBytes, w, lane);
     for (int c = 0; c < n_chunks; ++c) {
         wait_loads(w, c + 1 < n_chunks);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
         __builtin_amdgcn_s_barrier();        // every wave's chunk-c loads have landed
         if (c + 2 < n_chunks)
-            issue_stage<Half>(split, i0, j0, n_frames, n_chunks, c + 2,
-                              lds + ((c + 2) % kStages) * kStageBytes, w, lane);
+            issue_stage<Half>(g, i0, j0, n_chunks, c + 2, lds + ((c + 2) % kStages) * kStageBytes,
+                              w, lane);
         if (!active) continue;
         const char* st = lds + (c % kStages) * kStageBytes;
         vec b[6];  // planes q = 2 axis + part of this wave's 16 columns
@@ -384,17 +410,20 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
-                    acc[0][3 * r + cc][rb] = Mfma<Half>::run(a[rb][0], b[2 * cc], acc[0][3 * r + cc][rb]);
+                    acc[0][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][0], b[2 * cc], acc[0][3 * r + cc][rb]);
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
-                    acc[X][3 * r + cc][rb] = Mfma<Half>::run(a[rb][0], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
+                    acc[X][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][0], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
-                    acc[X][3 * r + cc][rb] = Mfma<Half>::run(a[rb][1], b[2 * cc], acc[X][3 * r + cc][rb]);
+                    acc[X][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][1], b[2 * cc], acc[X][3 * r + cc][rb]);
             if constexpr (kExact) {
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc)
@@ -414,7 +443,7 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
     // 2,048 pairs of the round, four each.  One copy of the FP64 QCP code and
     // no accumulator live while it runs.
     float* ep = reinterpret_cast<float*>(lds);
-    const double n = (double)n_sel;
+    const double n = (double)g.n_sel;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
         if (active) {
@@ -432,11 +461,12 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
             const int p = threadIdx.x + kMBlock * k;  // 0..2047
             const int pw = p >> 8, v = (p >> 6) & 3, ln = p & 63;
             const int pwi = pw >> 2, pwj = pw & 3;
-            if (ti == tj && 32 * pwi >= 16 * pwj + 16) continue;  // inactive wave
+            if (!kRect && ti == tj && 32 * pwi >= 16 * pwj + 16) continue;  // inactive wave
             const long long i = i0 + 32 * pwi + 16 * rb + 4 * (ln >> 4) + v;
             const long long j = j0 + 16 * pwj + (ln & 15);
-            if (i < j && j < n_frames) {
-                const double sij = (double)(scale[i] * scale[j]);
+            const bool valid = kRect ? (i < g.n_a && j < g.n_b) : (i < j && j < g.n_a);
+            if (valid) {
+                const double sij = (double)(g.scale_a[i] * g.scale_b[j]);
                 double M[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) {
@@ -445,8 +475,9 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(
                         val += (double)ep[(((pw * kSets + 1) * 9 + e) * 4 + v) * 64 + ln];
                     M[e] = val * sij;
                 }
-                const double msd = qcp_msd(M, traces[i], traces[j], n);
-                out[condensed_index(i, j, n_frames)] = (float)sqrt(msd);
+                const double msd = qcp_msd(M, g.traces_a[i], g.traces_b[j], n);
+                const long long o = kRect ? i * g.out_ld + j : condensed_index(i, j, g.n_a);
+                g.out[o] = (float)sqrt(msd);
             }
         }
         __syncthreads();
@@ -466,70 +497,123 @@ int hi_bits_for(int k_pad) {
     return b > 7 ? 7 : b;
 }
 
-// Workspace layout (every piece 256-byte aligned):
-//   traces F doubles | planes F x 3 x K floats |
-//   split F x 6 x K halfs | scale F floats | traces_split F doubles  (split modes)
-struct PairsWorkspace {
+// Prepared split block of F frames (every piece 256-byte aligned):
+//   traces F doubles | scale F floats | split F x 6 x K halfs
+struct SplitBlock {
     double* traces;
-    float* planes;
-    void* split;
     float* scale;
-    double* traces_split;
+    void* split;
     size_t bytes;
 };
 
-PairsWorkspace carve(char* base, long long F, int K, bool split) {
-    PairsWorkspace w{};
+SplitBlock carve_split(char* base, long long F, int K) {
+    SplitBlock b{};
     size_t off = 0;
-    w.traces = reinterpret_cast<double*>(base + off);
+    b.traces = reinterpret_cast<double*>(base + off);
     off += align256((size_t)F * sizeof(double));
-    w.planes = reinterpret_cast<float*>(base + off);
-    off += align256((size_t)F * 3 * K * sizeof(float));
-    if (split) {
-        w.split = base + off;
-        off += align256((size_t)F * 6 * K * sizeof(uint16_t));
-        w.scale = reinterpret_cast<float*>(base + off);
-        off += align256((size_t)F * sizeof(float));
-        w.traces_split = reinterpret_cast<double*>(base + off);
-        off += align256((size_t)F * sizeof(double));
-    }
-    w.bytes = off;
-    return w;
+    b.scale = reinterpret_cast<float*>(base + off);
+    off += align256((size_t)F * sizeof(float));
+    b.split = base + off;
+    off += align256((size_t)F * 6 * K * sizeof(uint16_t));
+    b.bytes = off;
+    return b;
+}
+
+// FP32-mode workspace: traces F doubles | planes F x 3 x K floats.
+size_t fp32_ws_bytes(long long F, int K) {
+    return align256((size_t)F * sizeof(double)) + align256((size_t)F * 3 * K * sizeof(float));
+}
+
+int prep_blocks_for(long long F) {
+    const DeviceInfo* d = current_device();
+    long long pw = (long long)(d ? d->cu_count : 256) * 16;
+    if (pw > F) pw = F;
+    return (int)((pw + 3) / 4);
 }
 
 template <typename Half, Split kSplit>
-int launch_split(hipStream_t s, const PairsWorkspace& w, const float* xyz, long long F,
-                 int n_atoms_total, const int* idx, int n_sel, int K, int prep_blocks,
-                 long long n_tiles, float* out) {
+int launch_prep(hipStream_t s, const float* xyz, long long F, int n_atoms_total, const int* idx,
+                int n_sel, int K, const SplitBlock& b) {
     const int bits = hi_bits_for(K);
     if (kSplit == Split::ExactHi && bits < 4) {
         set_error("selection of %d atoms too large for exact hi.hi accumulation", n_sel);
         return HR_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((pairs_prep_kernel<Half, kSplit>), dim3(prep_blocks), dim3(kPrepBlock), 0,
-                       s, xyz, F, n_atoms_total, idx, n_sel, K, bits, w.planes, w.traces,
-                       static_cast<Half*>(w.split), w.scale, w.traces_split);
-    int rc = launch_status("pairs_prep_kernel");
-    if (rc) return rc;
-    const long long T = (F + kMTile - 1) / kMTile;
-    const long long TS = (T + kSuper - 1) / kSuper;
-    const long long n_blocks = TS * (TS + 1) / 2 * kSuper * kSuper;
-    if (n_blocks >= (1LL << 31)) {
-        set_error("too many tiles (%lld)", n_blocks);
-        return HR_ERR_UNSUPPORTED;
-    }
-    (void)n_tiles;
+    hipLaunchKernelGGL((pairs_prep_kernel<Half, kSplit>), dim3(prep_blocks_for(F)),
+                       dim3(kPrepBlock), 0, s, xyz, F, n_atoms_total, idx, n_sel, K, bits,
+                       static_cast<float*>(nullptr), static_cast<double*>(nullptr),
+                       static_cast<Half*>(b.split), b.scale, b.traces);
+    return launch_status("pairs_prep_kernel");
+}
+
+template <typename Half, Split kSplit, bool kRect>
+int launch_gram(hipStream_t s, GramArgs<Half> g) {
     static bool lds_attr_set = false;
     if (!lds_attr_set) {
         HR_CHECK_HIP(hipFuncSetAttribute(
-            reinterpret_cast<const void*>(pairs_mfma_kernel<Half, kSplit>),
+            reinterpret_cast<const void*>(pairs_mfma_kernel<Half, kSplit, kRect>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfmaLds));
         lds_attr_set = true;
     }
-    hipLaunchKernelGGL((pairs_mfma_kernel<Half, kSplit>), dim3((unsigned)n_blocks), dim3(kMBlock),
-                       kMfmaLds, s, static_cast<const Half*>(w.split), w.scale, w.traces_split,
-                       F, n_sel, K, n_blocks, out);
+    hipLaunchKernelGGL((pairs_mfma_kernel<Half, kSplit, kRect>), dim3((unsigned)g.n_blocks),
+                       dim3(kMBlock), kMfmaLds, s, g);
     return launch_status("pairs_mfma_kernel");
+}
+
+template <typename Half, Split kSplit>
+int all_pairs_split(hipStream_t s, void* ws, const float* xyz, long long F, int n_atoms_total,
+                    const int* idx, int n_sel, int K, float* out) {
+    SplitBlock b = carve_split(static_cast<char*>(ws), F, K);
+    int rc = launch_prep<Half, kSplit>(s, xyz, F, n_atoms_total, idx, n_sel, K, b);
+    if (rc) return rc;
+    const long long T = (F + kMTile - 1) / kMTile;
+    const long long TS = (T + kSuper - 1) / kSuper;
+    GramArgs<Half> g{};
+    g.split_a = g.split_b = static_cast<const Half*>(b.split);
+    g.scale_a = g.scale_b = b.scale;
+    g.traces_a = g.traces_b = b.traces;
+    g.n_a = g.n_b = F;
+    g.n_sel = n_sel;
+    g.k_pad = K;
+    g.n_blocks = TS * (TS + 1) / 2 * kSuper * kSuper;
+    g.out = out;
+    if (g.n_blocks >= (1LL << 31)) {
+        set_error("too many tiles (%lld)", g.n_blocks);
+        return HR_ERR_UNSUPPORTED;
+    }
+    return launch_gram<Half, kSplit, false>(s, g);
+}
+
+template <typename Half, Split kSplit>
+int cross_split(hipStream_t s, const void* a, long long n_a, const void* b, long long n_b, int n_sel,
+                int K, float* out, long long out_ld) {
+    SplitBlock ba = carve_split(static_cast<char*>(const_cast<void*>(a)), n_a, K);
+    SplitBlock bb = carve_split(static_cast<char*>(const_cast<void*>(b)), n_b, K);
+    GramArgs<Half> g{};
+    g.split_a = static_cast<const Half*>(ba.split);
+    g.split_b = static_cast<const Half*>(bb.split);
+    g.scale_a = ba.scale;
+    g.scale_b = bb.scale;
+    g.traces_a = ba.traces;
+    g.traces_b = bb.traces;
+    g.n_a = n_a;
+    g.n_b = n_b;
+    g.n_sel = n_sel;
+    g.k_pad = K;
+    g.tiles_b = (n_b + kMTile - 1) / kMTile;
+    g.n_blocks = (n_a + kMTile - 1) / kMTile * g.tiles_b;
+    g.out = out;
+    g.out_ld = out_ld;
+    if (g.n_blocks >= (1LL << 31)) {
+        set_error("too many tiles (%lld)", g.n_blocks);
+        return HR_ERR_UNSUPPORTED;
+    }
+    return launch_gram<Half, kSplit, true>(s, g);
+}
+
+bool split_mode(int mode) {
+    return mode == HR_PAIRS_F16X3 || mode == HR_PAIRS_BF16X3 || mode == HR_PAIRS_F16_EXACT ||
+           mode == HR_PAIRS_BF16_EXACT;
 }
 
 }  // namespace
@@ -541,7 +625,9 @@ extern "C" {
 
 size_t hr_rmsd_all_pairs_workspace_bytes(int64_t n_frames, int32_t n_sel, int32_t mode) {
     if (n_frames <= 0 || n_sel <= 0) return 256;
-    return carve(nullptr, n_frames, k_pad_for(n_sel), mode != HR_PAIRS_FP32).bytes;
+    const int K = k_pad_for(n_sel);
+    if (mode == HR_PAIRS_FP32) return fp32_ws_bytes(n_frames, K);
+    return carve_split(nullptr, n_frames, K).bytes;
 }
 
 int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
@@ -553,53 +639,114 @@ int hr_rmsd_all_pairs(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
                n_atoms_total, n_sel);
     HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
                "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
-    HR_REQUIRE(mode >= HR_PAIRS_FP32 && mode <= HR_PAIRS_BF16_EXACT, "unknown mode %d", mode);
+    HR_REQUIRE(mode == HR_PAIRS_FP32 || split_mode(mode), "unknown mode %d", mode);
     if (n_frames < 2) return HR_OK;
     HR_REQUIRE(xyz && out && workspace, "NULL pointer argument");
     HR_REQUIRE(workspace_bytes >= hr_rmsd_all_pairs_workspace_bytes(n_frames, n_sel, mode),
                "workspace too small: %zu < %zu", workspace_bytes,
                hr_rmsd_all_pairs_workspace_bytes(n_frames, n_sel, mode));
-    const DeviceInfo* d = current_device();
-    if (!d) return HR_ERR_HIP;
+    if (!current_device()) return HR_ERR_HIP;
     hipStream_t s = (hipStream_t)stream;
     const int K = k_pad_for(n_sel);
-    const bool split = mode != HR_PAIRS_FP32;
-    PairsWorkspace w = carve(static_cast<char*>(workspace), n_frames, K, split);
-
-    long long pw = (long long)d->cu_count * 16;
-    if (pw > n_frames) pw = n_frames;
-    const int prep_blocks = (int)((pw + 3) / 4);
-    const int tile = split ? kMTile : kF32Tile;
-    const long long T = (n_frames + tile - 1) / tile;
-    const long long n_tiles = T * (T + 1) / 2;
-    HR_REQUIRE(n_tiles < (1LL << 31), "too many tiles (%lld)", n_tiles);
     const long long F = n_frames;
     switch (mode) {
         case HR_PAIRS_F16X3:
-            return launch_split<_Float16, Split::ThreeProduct>(s, w, xyz, F, n_atoms_total,
-                                                               atom_idx, n_sel, K, prep_blocks,
-                                                               n_tiles, out);
+            return all_pairs_split<_Float16, Split::ThreeProduct>(s, workspace, xyz, F,
+                                                                  n_atoms_total, atom_idx, n_sel,
+                                                                  K, out);
         case HR_PAIRS_BF16X3:
-            return launch_split<__bf16, Split::ThreeProduct>(s, w, xyz, F, n_atoms_total,
-                                                             atom_idx, n_sel, K, prep_blocks,
-                                                             n_tiles, out);
+            return all_pairs_split<__bf16, Split::ThreeProduct>(s, workspace, xyz, F,
+                                                                n_atoms_total, atom_idx, n_sel, K,
+                                                                out);
         case HR_PAIRS_F16_EXACT:
-            return launch_split<_Float16, Split::ExactHi>(s, w, xyz, F, n_atoms_total, atom_idx,
-                                                          n_sel, K, prep_blocks, n_tiles, out);
+            return all_pairs_split<_Float16, Split::ExactHi>(s, workspace, xyz, F, n_atoms_total,
+                                                             atom_idx, n_sel, K, out);
         case HR_PAIRS_BF16_EXACT:
-            return launch_split<__bf16, Split::ExactHi>(s, w, xyz, F, n_atoms_total, atom_idx,
-                                                        n_sel, K, prep_blocks, n_tiles, out);
+            return all_pairs_split<__bf16, Split::ExactHi>(s, workspace, xyz, F, n_atoms_total,
+                                                           atom_idx, n_sel, K, out);
         default:
             break;
     }
-    hipLaunchKernelGGL((pairs_prep_kernel<_Float16, Split::ThreeProduct>), dim3(prep_blocks),
-                       dim3(kPrepBlock), 0, s, xyz, F, n_atoms_total, atom_idx, n_sel, K, 0,
-                       w.planes, w.traces, static_cast<_Float16*>(nullptr), nullptr, nullptr);
+    char* ws = static_cast<char*>(workspace);
+    double* traces = reinterpret_cast<double*>(ws);
+    float* planes = reinterpret_cast<float*>(ws + align256((size_t)F * sizeof(double)));
+    hipLaunchKernelGGL((pairs_prep_kernel<_Float16, Split::ThreeProduct>), dim3(prep_blocks_for(F)),
+                       dim3(kPrepBlock), 0, s, xyz, F, n_atoms_total, atom_idx, n_sel, K, 0, planes,
+                       traces, static_cast<_Float16*>(nullptr), nullptr, nullptr);
     int rc = launch_status("pairs_prep_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL(pairs_fp32_kernel, dim3((unsigned)n_tiles), dim3(kF32Block), 0, s,
-                       w.planes, w.traces, F, n_sel, K, n_tiles, out);
+    const long long T = (F + kF32Tile - 1) / kF32Tile;
+    const long long n_tiles = T * (T + 1) / 2;
+    HR_REQUIRE(n_tiles < (1LL << 31), "too many tiles (%lld)", n_tiles);
+    hipLaunchKernelGGL(pairs_fp32_kernel, dim3((unsigned)n_tiles), dim3(kF32Block), 0, s, planes,
+                       traces, F, n_sel, K, n_tiles, out);
     return launch_status("pairs_fp32_kernel");
+}
+
+size_t hr_split_bytes(int64_t n_frames, int32_t n_sel, int32_t mode) {
+    if (n_frames <= 0 || n_sel <= 0 || !split_mode(mode)) return 0;
+    return carve_split(nullptr, n_frames, k_pad_for(n_sel)).bytes;
+}
+
+int hr_split_prepare(const float* xyz, int64_t n_frames, int32_t n_atoms_total,
+                     const int32_t* atom_idx, int32_t n_sel, int32_t mode, void* block,
+                     size_t block_bytes, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_atoms_total > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_sel=%d", (long long)n_frames,
+               n_atoms_total, n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms_total,
+               "n_sel (%d) must equal n_atoms_total (%d) without atom_idx", n_sel, n_atoms_total);
+    HR_REQUIRE(split_mode(mode), "mode %d is not a split mode", mode);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(xyz && block, "NULL pointer argument");
+    HR_REQUIRE(block_bytes >= hr_split_bytes(n_frames, n_sel, mode), "block too small");
+    if (!current_device()) return HR_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const int K = k_pad_for(n_sel);
+    SplitBlock b = carve_split(static_cast<char*>(block), n_frames, K);
+    switch (mode) {
+        case HR_PAIRS_F16X3:
+            return launch_prep<_Float16, Split::ThreeProduct>(s, xyz, n_frames, n_atoms_total,
+                                                              atom_idx, n_sel, K, b);
+        case HR_PAIRS_BF16X3:
+            return launch_prep<__bf16, Split::ThreeProduct>(s, xyz, n_frames, n_atoms_total,
+                                                            atom_idx, n_sel, K, b);
+        case HR_PAIRS_F16_EXACT:
+            return launch_prep<_Float16, Split::ExactHi>(s, xyz, n_frames, n_atoms_total, atom_idx,
+                                                         n_sel, K, b);
+        default:
+            return launch_prep<__bf16, Split::ExactHi>(s, xyz, n_frames, n_atoms_total, atom_idx,
+                                                       n_sel, K, b);
+    }
+}
+
+int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t n_b,
+                  int32_t n_sel, int32_t mode, float* out, int64_t out_ld, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_a >= 0 && n_b >= 0 && n_sel > 0 && out_ld >= n_b,
+               "invalid sizes n_a=%lld n_b=%lld n_sel=%d out_ld=%lld", (long long)n_a,
+               (long long)n_b, n_sel, (long long)out_ld);
+    HR_REQUIRE(split_mode(mode), "mode %d is not a split mode", mode);
+    if (n_a == 0 || n_b == 0) return HR_OK;
+    HR_REQUIRE(block_a && block_b && out, "NULL pointer argument");
+    if (!current_device()) return HR_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const int K = k_pad_for(n_sel);
+    switch (mode) {
+        case HR_PAIRS_F16X3:
+            return cross_split<_Float16, Split::ThreeProduct>(s, block_a, n_a, block_b, n_b, n_sel,
+                                                              K, out, out_ld);
+        case HR_PAIRS_BF16X3:
+            return cross_split<__bf16, Split::ThreeProduct>(s, block_a, n_a, block_b, n_b, n_sel,
+                                                            K, out, out_ld);
+        case HR_PAIRS_F16_EXACT:
+            return cross_split<_Float16, Split::ExactHi>(s, block_a, n_a, block_b, n_b, n_sel, K,
+                                                         out, out_ld);
+        default:
+            return cross_split<__bf16, Split::ExactHi>(s, block_a, n_a, block_b, n_b, n_sel, K,
+                                                       out, out_ld);
+    }
 }
 
 }  // extern "C"

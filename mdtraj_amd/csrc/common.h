@@ -90,33 +90,63 @@ __device__ __forceinline__ double seg_sum(double v, int width) {
 // Largest real root of P(x) = x^4 + c2 x^2 + c1 x + c0 whose roots are all
 // real (P is the characteristic polynomial of the symmetric key matrix K).
 // Newton from a start that lies above every root converges monotonically to
-// the largest root (Theobald, Acta Cryst. A61:478, 2005).  The start is
-// e0 = (G_a + G_b) / 2, an upper bound of lambda_max; if rounding puts it
-// below (some derivative of P negative there), the Frobenius bound
-// ||K||_F = 2 ||M||_F is used instead.
-__device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
-                                          double m_frob) {
-    double x = e0;
-    {
-        const double x2 = x * x;
-        const double p = (x2 + c2) * x2 + c1 * x + c0;
-        const double d1 = (4.0 * x2 + 2.0 * c2) * x + c1;
-        const double d2 = 12.0 * x2 + 2.0 * c2;
-        if (!(p >= 0.0 && d1 >= 0.0 && d2 >= 0.0 && x >= 0.0))
-            x = 2.0 * m_frob * (1.0 + 1e-12) + 1e-300;
+// the largest root (Theobald, Acta Cryst. A61:478, 2005).  A point lies
+// above every root iff P, its first and second derivatives and x (the third
+// derivative is 24 x) are all positive there (Budan-Fourier); the start
+// e0 = (G_a + G_b) / 2 is such a point in exact arithmetic, and when rounding
+// says otherwise the Frobenius bound ||K||_F = 2 ||M||_F is used.  Every
+// Newton iterate is checked the same way and the iteration stops at the last
+// one that passes, so rounding noise near a (near-)double root can never
+// carry it below lambda_max.
+//
+// The result is one more Newton step from the last such point x,
+// y = x - P/P'.  With h = x - lambda_max: P'/P = sum_i 1/(x - lambda_i)
+// <= 4/h gives h <= 4 P/P'; Taylor at x with P'' increasing on x > 0 (its
+// derivative is 24 x) gives P' h <= P + P''(x) h^2 / 2, and together
+// lambda_max lies in [y - 8 P'' P^2 / P'^3, y].  *resolved says whether that
+// bracket, widened by the rounding bound of P, is within 1e-11 x.  For a
+// simple root it is (the bracket is quadratic in the last step); it is not at
+// (near-)double roots -- the two largest eigenvalues of K (nearly) coincide,
+// as for a centred two-atom (collinear) structure -- where P's values
+// determine the root only to about sqrt(eps) lambda.
+__host__ __device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
+                                                   double m_frob, bool* resolved) {
+    double x = e0, p, d1, d2;
+    auto eval = [&](double t, double& pp, double& q1, double& q2) {
+        const double t2 = t * t;
+        pp = (t2 + c2) * t2 + c1 * t + c0;
+        q1 = (4.0 * t2 + 2.0 * c2) * t + c1;
+        q2 = 12.0 * t2 + 2.0 * c2;
+    };
+    eval(x, p, d1, d2);
+    if (!(p > 0.0 && d1 > 0.0 && d2 > 0.0 && x > 0.0)) {
+        x = 2.0 * m_frob * (1.0 + 1e-12) + 1e-300;
+        eval(x, p, d1, d2);
+        if (!(p > 0.0 && d1 > 0.0 && d2 > 0.0 && x > 0.0)) {
+            *resolved = false;
+            return x;
+        }
     }
     for (int it = 0; it < 64; ++it) {
-        const double x2 = x * x;
-        const double p = (x2 + c2) * x2 + c1 * x + c0;
-        const double d1 = (4.0 * x2 + 2.0 * c2) * x + c1;
-        if (!(p > 0.0) || !(d1 > 0.0)) break;
         const double nx = x - p / d1;
         if (!(nx < x)) break;
+        double np, nd1, nd2;
+        eval(nx, np, nd1, nd2);
+        if (!(np > 0.0 && nd1 > 0.0 && nd2 > 0.0 && nx > 0.0)) break;
         const double step = x - nx;
         x = nx;
+        p = np;
+        d1 = nd1;
+        d2 = nd2;
         if (step <= 1e-15 * x) break;
     }
-    return x;
+    const double x2 = x * x;
+    // Horner rounding bound of P at x: 4 eps times the sum of |terms|
+    const double perr = 8.9e-16 * (x2 * x2 + fabs(c2) * x2 + fabs(c1) * x + fabs(c0));
+    const double q = p + perr;
+    const double width = 8.0 * d2 * q * q / (d1 * d1 * d1) + 2.0 * perr / d1;
+    *resolved = width <= 1e-11 * x;
+    return x - p / d1;
 }
 
 // Key-matrix entries of Horn/Theobald for M[r*3+c] = sum a_r b_c, in the
@@ -127,7 +157,7 @@ struct KeyMatrix {
     double k00, k01, k02, k03, k11, k12, k13, k22, k23, k33;
 };
 
-__device__ __forceinline__ KeyMatrix key_matrix(const double M[9]) {
+__host__ __device__ __forceinline__ KeyMatrix key_matrix(const double M[9]) {
     KeyMatrix k;
     k.k00 = M[0] + M[4] + M[8];
     k.k01 = M[7] - M[5];
@@ -142,52 +172,100 @@ __device__ __forceinline__ KeyMatrix key_matrix(const double M[9]) {
     return k;
 }
 
-// lambda_max of K(M) in FP64.  Coefficients of the characteristic
-// polynomial: C2 = -2 |M|_F^2, C1 = -8 det M, C0 = det K.
-__device__ inline double qcp_lambda(const double M[9], double e0, KeyMatrix* kout) {
-    const KeyMatrix k = key_matrix(M);
-    double mm = 0.0;
+// One Jacobi rotation of the symmetric 4x4 a in the (p, r) plane that zeroes
+// a[p][r] (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5.2, with
+// the smaller rotation angle); v accumulates the rotations when kVec.
+template <int p, int r, bool kVec>
+__host__ __device__ __forceinline__ void jacobi_rotate(double (&a)[4][4], double (&v)[4][4]) {
+    const double apr = a[p][r];
+    if (apr == 0.0) return;
+    const double theta = (a[r][r] - a[p][p]) / (2.0 * apr);
+    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+    const double c = 1.0 / sqrt(fma(t, t, 1.0));
+    const double s = t * c;
+    a[p][p] -= t * apr;
+    a[r][r] += t * apr;
+    a[p][r] = 0.0;
+    a[r][p] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) mm = fma(M[i], M[i], mm);
-    const double c2 = -2.0 * mm;
-    const double det_m = M[0] * (M[4] * M[8] - M[5] * M[7]) -
-                         M[1] * (M[3] * M[8] - M[5] * M[6]) +
-                         M[2] * (M[3] * M[7] - M[4] * M[6]);
-    const double c1 = -8.0 * det_m;
-    // det K by cofactor expansion along the first row
-    const double a = k.k22 * k.k33 - k.k23 * k.k23;
-    const double b = k.k12 * k.k33 - k.k13 * k.k23;
-    const double c = k.k12 * k.k23 - k.k13 * k.k22;
-    const double d = k.k02 * k.k33 - k.k03 * k.k23;
-    const double e = k.k02 * k.k23 - k.k03 * k.k22;
-    const double f = k.k02 * k.k13 - k.k03 * k.k12;
-    const double c0 = k.k00 * (k.k11 * a - k.k12 * b + k.k13 * c) -
-                      k.k01 * (k.k01 * a - k.k12 * d + k.k13 * e) +
-                      k.k02 * (k.k01 * b - k.k11 * d + k.k13 * f) -
-                      k.k03 * (k.k01 * c - k.k11 * e + k.k12 * f);
-    if (kout) *kout = k;
-    return qcp_largest_root(c2, c1, c0, e0, sqrt(mm));
+    for (int m = 0; m < 4; ++m) {
+        if (m == p || m == r) continue;
+        const double g = a[m][p], h = a[m][r];
+        a[m][p] = a[p][m] = c * g - s * h;
+        a[m][r] = a[r][m] = s * g + c * h;
+    }
+    if constexpr (kVec) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double g = v[m][p], h = v[m][r];
+            v[m][p] = c * g - s * h;
+            v[m][r] = s * g + c * h;
+        }
+    }
 }
 
-// MSD from the centred correlation matrix and traces, all FP64.
-__device__ __forceinline__ double qcp_msd(const double M[9], double ga, double gb, double n) {
-    const double e0 = 0.5 * (ga + gb);
-    const double lam = qcp_lambda(M, e0, nullptr);
-    const double msd = (ga + gb - 2.0 * lam) / n;
-    return msd > 0.0 ? msd : 0.0;
+// lambda_max of K by cyclic Jacobi: absolute error a few eps ||K|| whatever
+// the eigenvalue spacing.  With kVec, q receives a unit eigenvector of it.
+// Used only where Newton on P cannot resolve the root (see above).
+template <bool kVec>
+__host__ __device__ inline double qcp_jacobi(const KeyMatrix& k, double q[4]) {
+    double a[4][4] = {{k.k00, k.k01, k.k02, k.k03},
+                      {k.k01, k.k11, k.k12, k.k13},
+                      {k.k02, k.k12, k.k22, k.k23},
+                      {k.k03, k.k13, k.k23, k.k33}};
+    double v[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[0][3] * a[0][3] +
+                           a[1][2] * a[1][2] + a[1][3] * a[1][3] + a[2][3] * a[2][3];
+        const double diag =
+            a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2] + a[3][3] * a[3][3];
+        if (!(off > 1e-34 * diag)) break;
+        jacobi_rotate<0, 1, kVec>(a, v);
+        jacobi_rotate<0, 2, kVec>(a, v);
+        jacobi_rotate<0, 3, kVec>(a, v);
+        jacobi_rotate<1, 2, kVec>(a, v);
+        jacobi_rotate<1, 3, kVec>(a, v);
+        jacobi_rotate<2, 3, kVec>(a, v);
+    }
+    int top = 0;
+    double lam = a[0][0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (a[i][i] > lam) { lam = a[i][i]; top = i; }
+    if constexpr (kVec) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            q[m] = top == 0 ? v[m][0] : top == 1 ? v[m][1] : top == 2 ? v[m][2] : v[m][3];
+    }
+    return lam;
 }
 
-__device__ __forceinline__ double det3(double a, double b, double c, double d, double e, double f,
-                                       double g, double h, double i) {
+__host__ __device__ __forceinline__ double det3(double a, double b, double c, double d, double e,
+                                                double f, double g, double h, double i) {
     return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
 }
 
-// Optimal rotation (row-major, applied as x' = x R) from K and lambda_max:
-// the top eigenvector of K is any non-zero column of adj(K - lambda I); the
-// column with the largest diagonal cofactor is used (the reference always
-// takes the first, rmsd/src/theobald_rmsd.cpp:279-331, and falls back to the
-// identity when it vanishes).  Identity if K - lambda I has rank < 3.
-__device__ inline void qcp_rotation(const KeyMatrix& k, double lam, double R[9]) {
+// Rotation matrix (row-major, applied as x' = x R) of the unit quaternion
+// (w, x, y, z) in the basis order of K.
+__host__ __device__ __forceinline__ void quat_rotation(double w, double x, double y, double z,
+                                                       double R[9]) {
+    R[0] = w * w + x * x - y * y - z * z;
+    R[1] = 2.0 * (x * y - w * z);
+    R[2] = 2.0 * (x * z + w * y);
+    R[3] = 2.0 * (x * y + w * z);
+    R[4] = w * w - x * x + y * y - z * z;
+    R[5] = 2.0 * (y * z - w * x);
+    R[6] = 2.0 * (x * z - w * y);
+    R[7] = 2.0 * (y * z + w * x);
+    R[8] = w * w - x * x - y * y + z * z;
+}
+
+// Optimal rotation from K and a simple lambda_max: the top eigenvector of K is
+// any non-zero column of adj(K - lambda I); the column with the largest
+// diagonal cofactor is used (the reference always takes the first,
+// rmsd/src/theobald_rmsd.cpp:279-331, and falls back to the identity when it
+// vanishes).  Returns false (R untouched) if K - lambda I has rank < 3.
+__host__ __device__ inline bool qcp_rotation(const KeyMatrix& k, double lam, double R[9]) {
     const double a00 = k.k00 - lam, a11 = k.k11 - lam, a22 = k.k22 - lam, a33 = k.k33 - lam;
     const double a01 = k.k01, a02 = k.k02, a03 = k.k03, a12 = k.k12, a13 = k.k13, a23 = k.k23;
     // Full adjugate of the symmetric 4x4 A (cofactors C_ij = (-1)^(i+j) det minor_ij).
@@ -209,23 +287,65 @@ __device__ inline void qcp_rotation(const KeyMatrix& k, double lam, double R[9])
     const double scale = fabs(a00) + fabs(a11) + fabs(a22) + fabs(a33) + fabs(a01) + fabs(a02) +
                          fabs(a03) + fabs(a12) + fabs(a13) + fabs(a23);
     const double s3 = scale * scale * scale;
-    if (!(qq > 1e-28 * s3 * s3) || !(qq > 1e-300)) {
-        R[0] = 1; R[1] = 0; R[2] = 0;
-        R[3] = 0; R[4] = 1; R[5] = 0;
-        R[6] = 0; R[7] = 0; R[8] = 1;
-        return;
-    }
+    if (!(qq > 1e-28 * s3 * s3) || !(qq > 1e-300)) return false;
     const double inv = 1.0 / sqrt(qq);
-    w *= inv; x *= inv; y *= inv; z *= inv;
-    R[0] = w * w + x * x - y * y - z * z;
-    R[1] = 2.0 * (x * y - w * z);
-    R[2] = 2.0 * (x * z + w * y);
-    R[3] = 2.0 * (x * y + w * z);
-    R[4] = w * w - x * x + y * y - z * z;
-    R[5] = 2.0 * (y * z - w * x);
-    R[6] = 2.0 * (x * z - w * y);
-    R[7] = 2.0 * (y * z + w * x);
-    R[8] = w * w - x * x - y * y + z * z;
+    quat_rotation(w * inv, x * inv, y * inv, z * inv, R);
+    return true;
+}
+
+// lambda_max of K(M) in FP64, and with kRot the optimal rotation R (identity
+// when K = 0).  Coefficients of the characteristic polynomial:
+// C2 = -2 |M|_F^2, C1 = -8 det M, C0 = det K.  Newton on P is the fast path;
+// an unresolved (near-double) root, or a rank-deficient K - lambda I for the
+// rotation, is settled by Jacobi on K itself.
+template <bool kRot>
+__host__ __device__ inline double qcp_solve(const double M[9], double e0, double R[9]) {
+    const KeyMatrix k = key_matrix(M);
+    double mm = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mm = fma(M[i], M[i], mm);
+    const double c2 = -2.0 * mm;
+    const double det_m = M[0] * (M[4] * M[8] - M[5] * M[7]) -
+                         M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                         M[2] * (M[3] * M[7] - M[4] * M[6]);
+    const double c1 = -8.0 * det_m;
+    // det K by cofactor expansion along the first row
+    const double a = k.k22 * k.k33 - k.k23 * k.k23;
+    const double b = k.k12 * k.k33 - k.k13 * k.k23;
+    const double c = k.k12 * k.k23 - k.k13 * k.k22;
+    const double d = k.k02 * k.k33 - k.k03 * k.k23;
+    const double e = k.k02 * k.k23 - k.k03 * k.k22;
+    const double f = k.k02 * k.k13 - k.k03 * k.k12;
+    const double c0 = k.k00 * (k.k11 * a - k.k12 * b + k.k13 * c) -
+                      k.k01 * (k.k01 * a - k.k12 * d + k.k13 * e) +
+                      k.k02 * (k.k01 * b - k.k11 * d + k.k13 * f) -
+                      k.k03 * (k.k01 * c - k.k11 * e + k.k12 * f);
+    bool resolved = true;
+    double lam = qcp_largest_root(c2, c1, c0, e0, sqrt(mm), &resolved);
+    if constexpr (kRot) {
+        if (resolved && qcp_rotation(k, lam, R)) return lam;
+        if (!(mm > 0.0)) {
+            quat_rotation(1.0, 0.0, 0.0, 0.0, R);
+            return 0.0;
+        }
+        double q[4];
+        lam = qcp_jacobi<true>(k, q);
+        const double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        const double inv = 1.0 / sqrt(qq);
+        quat_rotation(q[0] * inv, q[1] * inv, q[2] * inv, q[3] * inv, R);
+        return lam;
+    } else {
+        if (!resolved) lam = qcp_jacobi<false>(k, nullptr);
+        return lam;
+    }
+}
+
+// MSD from the centred correlation matrix and traces, all FP64.
+__host__ __device__ __forceinline__ double qcp_msd(const double M[9], double ga, double gb,
+                                                   double n) {
+    const double lam = qcp_solve<false>(M, 0.5 * (ga + gb), nullptr);
+    const double msd = (ga + gb - 2.0 * lam) / n;
+    return msd > 0.0 ? msd : 0.0;
 }
 
 }  // namespace hr

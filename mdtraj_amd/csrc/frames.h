@@ -312,10 +312,8 @@ __device__ __forceinline__ void process_group(
                 const double msd = qcp_msd(M, ga, gb, n);
                 out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
             } else {
-                KeyMatrix km;
-                const double lam = qcp_lambda(M, 0.5 * (ga + gb), &km);
                 double R[9];
-                qcp_rotation(km, lam, R);
+                const double lam = qcp_solve<true>(M, 0.5 * (ga + gb), R);
 #pragma unroll
                 for (int e = 0; e < 9; ++e) rot_out[9 * f + e] = (float)R[e];
                 mu_out[3 * f + 0] = mine.sx / n;

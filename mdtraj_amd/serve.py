@@ -1,0 +1,229 @@
+"""Serving one-vs-pool RMSD queries against device-resident trajectory pools.
+
+Pools are uploaded once and kept in HBM in prepared split form
+(``allpairs.SplitSet``: centred, hi/lo planes, traces).  A request carries 1 to
+64 query frames with the same atom count as its pool and asks for the RMSD of
+every query frame to every pool frame.
+
+Continuous batching: a scheduler thread takes the oldest pending request,
+gathers every other pending request for the same pool up to
+``max_batch_queries`` query frames, and launches the batch (upload, split
+prep, one cross-Gram pass over the pool, copy of the results to pinned host
+memory) on the next stream of a small ring as soon as a slot is free and the
+HBM budget admits its buffers.  Requests that arrive while earlier batches
+are on the GPU join later batches; batches on different streams overlap on
+the device.  A completion thread waits for each batch's event and hands every
+request its rows.  Each (query, pool frame) result depends only on those two
+frames (``hr_rmsd_cross``), so a request's answer is bit-identical whether
+it was scored alone or batched with others.
+"""
+from __future__ import annotations
+
+import collections
+import threading
+import time
+
+import numpy as np
+
+from . import _device as dev
+from ._lib import load
+from .allpairs import MODES, SplitSet, rmsd_cross
+
+
+class Request:
+    __slots__ = ("pool", "queries", "t_arrival", "t_done", "result", "_done")
+
+    def __init__(self, pool, queries, t_arrival):
+        self.pool = pool
+        self.queries = queries
+        self.t_arrival = t_arrival
+        self.t_done = None
+        self.result = None
+        self._done = threading.Event()
+
+    def wait(self, timeout=None):
+        if not self._done.wait(timeout):
+            raise TimeoutError("request not finished")
+        return self.result
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+class Pool:
+    def __init__(self, xyz, mode):
+        self.n_frames, self.n_atoms = int(xyz.shape[0]), int(xyz.shape[1])
+        self.split = SplitSet(xyz, mode=mode)
+        self.nbytes = int(self.split.nbytes)
+
+
+class RMSDServer:
+    """hbm_budget: bytes of HBM the server may hold (pools + batches in flight).
+    max_batch_queries: query frames per batch (<= 64 fills one MFMA row tile).
+    n_streams: batches that may be in flight at once."""
+
+    def __init__(self, hbm_budget=200e9, mode="f16exact", max_batch_queries=64, n_streams=3):
+        self.t = dev.require_gpu()
+        self.mode = mode
+        self.budget = int(hbm_budget)
+        self.max_q = int(max_batch_queries)
+        self.pools = []
+        self.resident = 0
+        self.inflight_bytes = 0
+        self.streams = [self.t.cuda.Stream() for _ in range(n_streams)]
+        self.free_streams = collections.deque(range(n_streams))
+        self.pending = collections.deque()
+        self.cv = threading.Condition()
+        self.completions = collections.deque()
+        self.done_cv = threading.Condition()
+        self.running = False
+        self.completed = []
+        self.pairs_done = 0
+        self.batches = 0
+
+    # ---------------------------------------------------------------- pools
+    def add_pool(self, xyz):
+        """Prepare a pool (float32 (F, N, 3), host or device) in HBM; returns its id.
+        Raises MemoryError when the HBM budget cannot hold it."""
+        pool = Pool(dev.to_device_f32(xyz), self.mode)
+        with self.cv:
+            if self.resident + pool.nbytes > self.budget:
+                raise MemoryError(f"pool of {pool.nbytes} bytes exceeds the HBM budget")
+            self.resident += pool.nbytes
+            self.pools.append(pool)
+            return len(self.pools) - 1
+
+    # -------------------------------------------------------------- requests
+    def submit(self, pool_id, queries, t_arrival=None):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 2:
+            q = q[None]
+        p = self.pools[pool_id]
+        if q.shape[1] != p.n_atoms or q.shape[2] != 3:
+            raise ValueError(f"queries must be (q, {p.n_atoms}, 3), got {q.shape}")
+        if not 1 <= q.shape[0] <= self.max_q:
+            raise ValueError(f"a request carries 1..{self.max_q} query frames")
+        r = Request(pool_id, q, time.perf_counter() if t_arrival is None else t_arrival)
+        with self.cv:
+            self.pending.append(r)
+            self.cv.notify_all()
+        return r
+
+    def _batch_bytes(self, pool, nq):
+        """HBM a batch of nq query frames holds while in flight: the uploaded
+        queries, their prepared split block and the float32 result rows."""
+        split = load().hr_split_bytes(nq, pool.n_atoms, MODES[self.mode])
+        return 12 * nq * pool.n_atoms + int(split) + 4 * nq * pool.n_frames
+
+    def _take_batch(self):
+        """Oldest pending request plus later ones for the same pool (FIFO
+        within the pool), up to max_q query frames."""
+        first = self.pending[0]
+        pool = self.pools[first.pool]
+        batch, nq = [], 0
+        keep = collections.deque()
+        while self.pending:
+            r = self.pending.popleft()
+            if r.pool == first.pool and nq + r.queries.shape[0] <= self.max_q:
+                batch.append(r)
+                nq += r.queries.shape[0]
+            else:
+                keep.append(r)
+        self.pending = keep
+        return pool, batch, nq
+
+    def _scheduler(self):
+        while True:
+            with self.cv:
+                while self.running and (not self.pending or not self.free_streams):
+                    self.cv.wait()
+                if not self.running and not self.pending:
+                    return
+                if not self.free_streams:
+                    continue
+                pool = self.pools[self.pending[0].pool]
+                need = self._batch_bytes(pool, self.max_q)
+                while self.running and self.resident + self.inflight_bytes + need > self.budget:
+                    self.cv.wait()
+                pool, batch, nq = self._take_batch()
+                si = self.free_streams.popleft()
+                self.inflight_bytes += need
+            self._launch(pool, batch, nq, si, need)
+
+    def _launch(self, pool, batch, nq, si, need):
+        t = self.t
+        stream = self.streams[si]
+        queries = np.concatenate([r.queries for r in batch], axis=0)
+        with t.cuda.stream(stream):
+            host_in = t.from_numpy(queries).pin_memory()
+            dq = host_in.to(device=dev.device(), non_blocking=True)
+            qs = SplitSet(dq, mode=self.mode)
+            out = rmsd_cross(qs, pool.split)
+            host_out = t.empty(out.shape, dtype=t.float32, pin_memory=True)
+            host_out.copy_(out, non_blocking=True)
+            ev = t.cuda.Event()
+            ev.record(stream)
+        with self.done_cv:
+            # keep device buffers alive until the event has passed
+            self.completions.append((ev, batch, host_out, (host_in, dq, qs, out), si, need, nq,
+                                     pool.n_frames))
+            self.done_cv.notify_all()
+
+    def _completer(self):
+        while True:
+            with self.done_cv:
+                while not self.completions and (self.running or self._sched.is_alive()):
+                    self.done_cv.wait(timeout=0.05)
+                if not self.completions:
+                    if not self.running and not self._sched.is_alive():
+                        return
+                    continue
+                ev, batch, host_out, keep, si, need, nq, nf = self.completions.popleft()
+            ev.synchronize()
+            now = time.perf_counter()
+            res = host_out.numpy()
+            row = 0
+            for r in batch:
+                q = r.queries.shape[0]
+                r.result = res[row:row + q]
+                row += q
+                r.t_done = now
+                r._done.set()
+            del keep
+            with self.cv:
+                self.free_streams.append(si)
+                self.inflight_bytes -= need
+                self.pairs_done += nq * nf
+                self.batches += 1
+                self.completed.extend(batch)
+                self.cv.notify_all()
+
+    # ------------------------------------------------------------- lifecycle
+    def start(self):
+        self.running = True
+        self._sched = threading.Thread(target=self._scheduler, daemon=True)
+        self._comp = threading.Thread(target=self._completer, daemon=True)
+        self._sched.start()
+        self._comp.start()
+        return self
+
+    def stop(self):
+        """Finish everything pending, then stop the threads."""
+        with self.cv:
+            self.running = False
+            self.cv.notify_all()
+        self._sched.join()
+        with self.done_cv:
+            self.done_cv.notify_all()
+        self._comp.join()
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def latency_percentiles(self, ps=(50, 99)):
+        lat = np.array([r.latency for r in self.completed if r.latency is not None])
+        return {p: float(np.percentile(lat, p)) if lat.size else None for p in ps}

@@ -37,3 +37,29 @@ def test_square_and_rows(gpu):
     assert D.shape == (50, 50) and np.all(np.diag(D) == 0) and np.array_equal(D, D.T)
     row = md.rmsd(md.Trajectory(x), md.Trajectory(x), 17)
     assert np.abs(D[17] - row).max() < TOL["fp32"]
+
+
+@pytest.mark.parametrize("mode", ["f16exact", "f16x3"])
+@pytest.mark.parametrize("QA,FB,N", [(1, 100, 40), (5, 300, 200), (64, 130, 500), (70, 65, 33)])
+def test_cross_vs_fp64(gpu, mode, QA, FB, N):
+    from mdtraj_amd.allpairs import SplitSet, rmsd_cross
+
+    x = gen.walk(seed=QA + FB, n_frames=QA + FB, n_atoms=N)
+    qa, pool = x[:QA], x[QA:]
+    got = rmsd_cross(SplitSet(qa, mode=mode), SplitSet(pool, mode=mode)).cpu().numpy()
+    D = fp64.rmsd_matrix(x)[:QA, QA:]
+    err = np.abs(got - D).max()
+    print(f"cross {mode} {QA}x{FB} N={N}: max err {err:.3e} nm")
+    assert err < TOL[mode]
+
+
+def test_cross_batch_independence(gpu):
+    """A query's row is bit-identical whether it is scored alone or with
+    other queries in the same call."""
+    from mdtraj_amd.allpairs import SplitSet, rmsd_cross
+
+    x = gen.walk(seed=77, n_frames=300, n_atoms=120)
+    pool = SplitSet(x[40:])
+    alone = rmsd_cross(SplitSet(x[7:8]), pool)
+    mixed = rmsd_cross(SplitSet(x[:40]), pool)
+    assert gpu.equal(alone[0], mixed[7])

@@ -87,6 +87,40 @@ def test_degenerate_and_tiny(gpu, golden):
     assert np.isfinite(md.rmsd(t2, t1, atom_indices=[1], ref_atom_indices=[0])).all()
 
 
+def _line_and_plane_frames(seed, n_frames, n_atoms, kind):
+    """Rigid copies (plus noise) of one collinear or planar structure."""
+    rng = np.random.default_rng(seed)
+    if kind == "line":
+        base = np.outer(rng.normal(size=n_atoms), rng.normal(size=3))
+    else:
+        base = rng.normal(size=(n_atoms, 3))
+        base[:, 2] = 0.0
+    base *= 2.0
+    q = gen.uniform_quaternion(rng, n_frames)
+    R = gen.rotation_matrix_from_quaternion(q)
+    noise = rng.choice([0.0, 1e-3, 3e-2], size=(n_frames, 1, 1))
+    x = np.einsum("na,fba->fnb", base, R) + noise * rng.normal(size=(n_frames, n_atoms, 3))
+    return (x + rng.normal(size=(n_frames, 1, 3))).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind,n_atoms", [("line", 2), ("line", 9), ("plane", 3), ("plane", 40)])
+def test_collinear_and_planar_vs_fp64(gpu, kind, n_atoms):
+    """Collinear structures give the key matrix a double top eigenvalue,
+    where Newton on its characteristic polynomial alone is only
+    sqrt(eps)-accurate; planar ones give det M = 0."""
+    x = _line_and_plane_frames(n_atoms, 150, n_atoms, kind)
+    t = T(x)
+    got = md.rmsd(t, t, 5)
+    o = fp64.rmsd(x, x, 5)
+    assert np.abs(got - o).max() < TOL_FP64
+    assert got[5] == 0.0
+    # superpose reaches the optimal RMSD even where the rotation is not unique
+    s = T(x.copy())
+    s.superpose(T(x), 5)
+    disp = np.sqrt(((s.xyz.astype(np.float64) - s.xyz[5]) ** 2).sum(2).mean(1))
+    assert np.abs(disp - o).max() < 2e-5
+
+
 def test_empty_target(gpu):
     t = T(np.zeros((0, 7, 3), np.float32))
     r = T(np.ones((1, 7, 3), np.float32))
