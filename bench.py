@@ -209,7 +209,13 @@ def bench_all_pairs(args):
     useful = 2.0 * (3 * F) ** 2 * N / 2   # upper triangle of the 3F x 3F Gram
     tflops = useful / (ms * 1e-3) / 1e12
     peak = FP32_PEAK_TFLOPS if args.mode == "fp32" else F16_PEAK_TFLOPS
-    return {
+    # MFMA products per useful product: 4 (exact-hi), 3 (three-product split)
+    products = {"fp32": 1, "f16x3": 3, "bf16x3": 3, "f16exact": 4, "bf16exact": 4}[args.mode]
+    k_pad = (N + 63) // 64 * 64
+    executed = products * useful * k_pad / N
+    log(f"all-pairs {args.mode}: {ms:.3f} ms, useful {tflops:.1f} TFLOP/s, executed "
+        f"{executed / (ms * 1e-3) / 1e12:.1f} TFLOP/s")
+    result = {
         "metric": METRIC,
         "value": args.steps * n_pairs / elapsed,
         "unit": "frame-pairs/s",
@@ -220,10 +226,44 @@ def bench_all_pairs(args):
                   "bf16exact": "bf16x4-exact-hi"}[args.mode],
         "data": "synthetic (seeded random walk generated in HBM)",
         "config": {"workload": f"all-pairs RMSD matrix, {F} frames x {N} atoms, mode {args.mode}",
-                   "kernel_ms": ms},
+                   "kernel_ms": ms, "mfma_products": products,
+                   "executed_tflops": executed / (ms * 1e-3) / 1e12,
+                   "executed_frac": executed / (ms * 1e-3) / 1e12 / peak},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": tflops / peak, "traffic": None},
     }
+    del out, ws
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_all_pairs(X, args.mode)
+    return result
+
+
+def cpu_baseline_all_pairs(X, mode_name, n_sub=2000):
+    """The FP64 oracle on a seeded 2,000-frame subset of the all-pairs
+    trajectory, timed on the host cores; on the same subset, the GPU's split
+    mode and the plain FP32 GPU kernel are checked against it."""
+    import numpy as np
+    import torch
+
+    from mdtraj_amd.allpairs import rmsd_matrix
+    from oracle import fp64
+
+    F = int(X.shape[0])
+    sub = np.sort(np.random.default_rng(2).choice(F, size=min(n_sub, F), replace=False))
+    Xs = X[torch.as_tensor(sub, device=X.device)].contiguous()
+    gpu = {m: rmsd_matrix(Xs, mode=m).cpu().numpy().astype(np.float64)
+           for m in sorted({mode_name, "fp32"})}
+    xs = Xs.cpu().numpy()
+    t0 = time.perf_counter()
+    want = fp64.condensed(fp64.rmsd_matrix(xs))
+    cpu_s = time.perf_counter() - t0
+    errs = {m: float(np.abs(v - want).max()) for m, v in gpu.items()}
+    log(f"all-pairs subset {len(sub)} frames: FP64 oracle {cpu_s:.1f} s; max err vs FP64 {errs}")
+    return {"value": want.size / cpu_s, "unit": "frame-pairs/s", "cores": cpu_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle (numpy) on a seeded {len(sub)}-frame subset, "
+                      f"{want.size} pairs",
+            "max_err_vs_fp64_nm": errs}
 
 
 # ------------------------------------------------------------------ batched
@@ -413,6 +453,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
                     help="register budget of the per-frame kernel (0 = library default)")
+    ap.add_argument("--frames-grid", type=int, default=0, choices=[0, 1, 2],
+                    help="work split of the per-frame kernel: 1 balanced quads, 2 one wave "
+                         "per resident slot (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -428,6 +471,7 @@ def main():
     from mdtraj_amd import _lib
 
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, args.frames_waves)
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_GRID, args.frames_grid)
     if args.workload == "hbm-ceiling":
         print(json.dumps(bench_hbm_ceiling(args)), flush=True)
         return

@@ -51,9 +51,15 @@ int hr_device_cu_count(void);
 /* Tuning knobs (no counterpart in the reference; set by the autotuner in
  * mdtraj_amd/autotune.py).  Values are process-wide; 0 restores the default.
  *   HR_TUNE_FRAMES_WAVES_PER_SIMD  register budget of the per-frame kernel:
- *                                  3 (default, no spills) or 4 waves/SIMD */
+ *                                  3 (default, no spills) or 4 waves/SIMD
+ *   HR_TUNE_FRAMES_GRID            work split of the per-frame kernel:
+ *                                  1 (default) whole frame quads, as many
+ *                                  waves as give every wave the same count;
+ *                                  2 every resident wave slot, frames split
+ *                                  evenly */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
-#define HR_TUNE_COUNT 1
+#define HR_TUNE_FRAMES_GRID 1
+#define HR_TUNE_COUNT 2
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 
@@ -147,6 +153,47 @@ int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ali
                  int32_t n_align_atoms, const int32_t* atom_idx, int32_t n_sel,
                  const float* ref_frame, const int32_t* ref_idx, int32_t translate,
                  float* rot_out, float* rmsd_out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Align/displace RMSD (replaces getMultipleAlignDisplaceRMSDs_atom_major,
+ * rmsd/_rmsd.pyx:677-759, which calls msd_atom_major(..., computeRot=1) and
+ * rot_msd_atom_major, rmsd/include/rotation.h:9-10): for every frame f, the
+ * rotation R_f that best fits align_ref onto align[f] (centroids removed;
+ * the reference expects centred alignment sets), then the RMS displacement
+ * of displ_ref R_f from displ[f], about the origin (no centering), FP64.
+ *   align       (n_frames, align_stride, 3), atoms 0..n_align-1 used
+ *   align_ref   (n_align, 3)
+ *   displ       (n_frames, displ_stride, 3), atoms 0..n_displ-1 used
+ *   displ_ref   (n_displ, 3)
+ *   rmsd_out    float32[n_frames]
+ *   rot_out     float32 (n_frames, 3, 3) R_f row-major (x' = x R), or NULL
+ *   workspace   hr_align_displace_workspace_bytes(n_frames, n_align) bytes
+ * ------------------------------------------------------------------------- */
+size_t hr_align_displace_workspace_bytes(int64_t n_frames, int32_t n_align);
+int hr_rmsd_align_displace(const float* align, int64_t n_frames, int32_t align_stride,
+                           int32_t n_align, const float* align_ref, const float* displ,
+                           int32_t displ_stride, int32_t n_displ, const float* displ_ref,
+                           float* rmsd_out, float* rot_out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Per-atom RMS fluctuation (replaces the kernel loops of md.rmsf,
+ * rmsd/_rmsd.pyx:246-443: msd_atom_major + rot_atom_major per frame, then
+ * the mean structure and the fluctuation about it).
+ *   xyz         (n_frames, n_atoms, 3); atom_idx int32[n_sel] or NULL (all)
+ *   ref_frame   (n_ref_atoms, 3) with ref_idx int32[n_sel] or NULL: every
+ *               frame's selection is fitted onto it and rotated by the fit;
+ *               ref_frame NULL: frames already aligned (no rotation)
+ *   center      1: the fluctuating coordinates are each frame's selection
+ *               centred on its centroid; 0: the selection as stored (rotated
+ *               about the origin when fitted), as md.rmsf does when it is
+ *               given atom_indices
+ *   out         float32[n_sel]: sqrt(mean_f |x_f - <x>|^2), FP64 sums
+ *   workspace   hr_rmsf_workspace_bytes(n_frames, n_sel) bytes
+ * ------------------------------------------------------------------------- */
+size_t hr_rmsf_workspace_bytes(int64_t n_frames, int32_t n_sel);
+int hr_rmsf(const float* xyz, int64_t n_frames, int32_t n_atoms, const int32_t* atom_idx,
+            int32_t n_sel, const float* ref_frame, const int32_t* ref_idx, int32_t center,
+            float* out, void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------
  * All-pairs RMSD matrix.  The reference has no such routine; its users loop

@@ -3,6 +3,8 @@
 Drop-in surface (names and semantics of ``mdtraj``):
     rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
          parallel=True, precentered=False, superpose=True)
+    rmsf(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
+         parallel=True, precentered=False, mode='atom')
     Trajectory(xyz).superpose(...), .center_coordinates(), ._rmsd_traces
     _rmsd: the ``mdtraj._rmsd`` module functions
     capi(): paths of the C header and shared library (as ``md.capi()``)
@@ -14,7 +16,7 @@ from __future__ import annotations
 import os
 
 from . import _rmsd
-from ._rmsd import rmsd
+from ._rmsd import rmsd, rmsf
 from .trajectory import Trajectory
 
 __version__ = "0.1.0"
@@ -34,4 +36,4 @@ def rmsd_matrix(*args, **kwargs):
     return _rm(*args, **kwargs)
 
 
-__all__ = ["rmsd", "Trajectory", "capi", "rmsd_matrix", "_rmsd"]
+__all__ = ["rmsd", "rmsf", "Trajectory", "capi", "rmsd_matrix", "_rmsd"]

@@ -15,6 +15,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hiprmsd.h")
 
 HR_OK = 0
 HR_TUNE_FRAMES_WAVES_PER_SIMD = 0
+HR_TUNE_FRAMES_GRID = 1
 
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1
@@ -44,6 +45,10 @@ _SIGNATURES = {
     "hr_rmsd_nosuperpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "hr_superpose_workspace_bytes": (_sz, [_i64, _i32]),
     "hr_superpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "hr_align_displace_workspace_bytes": (_sz, [_i64, _i32]),
+    "hr_rmsd_align_displace": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "hr_rmsf_workspace_bytes": (_sz, [_i64, _i32]),
+    "hr_rmsf": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
     "hr_rmsd_all_pairs_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "hr_rmsd_all_pairs": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _sz, _vp]),
     "hr_split_bytes": (_sz, [_i64, _i32, _i32]),

@@ -256,3 +256,141 @@ def superpose_atom_major(xyz_align_target, xyz_align_mobile, g_target, g_mobile,
         xyz_displace_mobile[...] = dev.to_host(D)
     elif D.data_ptr() != xyz_displace_mobile.data_ptr():
         xyz_displace_mobile.copy_(D)
+
+
+def rmsf(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
+         parallel=True, precentered=False, mode="atom"):
+    """Root-mean-square fluctuation of the selected atoms of ``target``.
+
+    Drop-in for ``md.rmsf`` (``rmsd/_rmsd.pyx:246-484``): unless ``reference``
+    is None (frames already aligned), each frame's selection is fitted onto
+    ``reference[frame]``'s and rotated by the fit; the fluctuation is taken
+    about the mean structure.  As in the reference, the fluctuating
+    coordinates are each frame's selection centred on its centroid when
+    ``atom_indices`` is None, and the selection as stored (rotated about the
+    origin) otherwise.  ``mode='residue'`` combines the atoms' mean-square
+    fluctuations per residue with mass weights (needs ``target.topology``
+    with ``atom(i).residue.index``, ``atom(i).element.mass`` and
+    ``n_residues``); residues without a selected atom read -1.  Deviations:
+    inputs are never modified (the reference centres ``target.xyz`` in place
+    when ``atom_indices`` is None); sums are FP64.
+    """
+    prealigned = reference is None
+    if reference is None:
+        reference = target
+    txyz = _xyz_of(target)
+    rxyz = _xyz_of(reference)
+    n_common = min(txyz.shape[1], rxyz.shape[1])
+    if atom_indices is not None:
+        atom_indices = _check_indices(atom_indices, n_common, "atom_indices")
+    if ref_atom_indices is None:
+        ref_atom_indices = atom_indices
+    else:
+        if atom_indices is None:
+            raise TypeError("ref_atom_indices requires atom_indices "
+                            "(the reference takes len() of the default selection)")
+        if len(ref_atom_indices) != len(atom_indices):
+            raise ValueError(
+                "atom_indices and ref_atom_indices must have same number of atom indices. "
+                "found %d and %d." % (len(atom_indices), len(ref_atom_indices)))
+        ref_atom_indices = _check_indices(ref_atom_indices, n_common, "ref_atom_indices")
+    if mode not in {"atom", "residue"}:
+        raise ValueError('Mode must be one of "atom" or "residue". "%s" supplied.' % mode)
+    assert (len(txyz.shape) == 3 and len(rxyz.shape) == 3
+            and txyz.shape[2] == 3 and rxyz.shape[2] == 3)
+    if atom_indices is None and txyz.shape[1] != rxyz.shape[1]:
+        raise ValueError("Input trajectories must have same number of atoms. "
+                         "found %d and %d." % (txyz.shape[1], rxyz.shape[1]))
+    if frame >= rxyz.shape[0]:
+        raise ValueError("Cannot calculate RMSF of frame %d: reference has "
+                         "only %d frames." % (frame, rxyz.shape[0]))
+    if frame < 0:
+        frame += rxyz.shape[0]
+    usable = (getattr(reference, "_rmsd_traces", None) is not None
+              and getattr(target, "_rmsd_traces", None) is not None and atom_indices is None)
+    if precentered and not usable:
+        warnings.warn("in rmsd(), precentered is ignored when atom_indices != None",
+                      RuntimeWarning)
+
+    lib = load()
+    on_device = dev.is_device_array(txyz)
+    T = dev.to_device_f32(txyz)
+    n_frames, n_atoms = int(T.shape[0]), int(T.shape[1])
+    n_sel = n_atoms if atom_indices is None else len(atom_indices)
+    out = dev.empty_f32(n_sel)
+    if n_frames == 0:
+        out.zero_()
+    else:
+        d_idx = dev.to_device_i32(atom_indices)
+        d_ridx = dev.to_device_i32(ref_atom_indices)
+        R = None if prealigned else dev.to_device_f32(rxyz[frame])
+        ws = dev.workspace(lib.hr_rmsf_workspace_bytes(n_frames, n_sel))
+        call("hr_rmsf", T.data_ptr(), n_frames, n_atoms, dev.ptr(d_idx), n_sel,
+             None if R is None else R.data_ptr(), dev.ptr(d_ridx), int(atom_indices is None),
+             out.data_ptr(), ws.data_ptr(), dev.stream())
+    if mode == "atom":
+        return _result(out, on_device)
+    top = target.topology
+    sel = np.arange(n_atoms) if atom_indices is None else np.asarray(
+        dev.to_host(atom_indices) if dev.is_device_array(atom_indices) else atom_indices)
+    residue_of = np.array([top.atom(int(i)).residue.index for i in sel], dtype=np.int64)
+    masses = np.array([float(top.atom(int(i)).element.mass) for i in sel], dtype=np.float64)
+    msf = dev.to_host(out).astype(np.float64) ** 2
+    num = np.zeros(top.n_residues)
+    den = np.zeros(top.n_residues)
+    np.add.at(num, residue_of, msf * masses)
+    np.add.at(den, residue_of, masses)
+    res = np.full(top.n_residues, -1.0, dtype=np.float64)
+    ok = den > 0
+    res[ok] = np.sqrt(num[ok] / den[ok])
+    return res.astype(np.float32)
+
+
+def getMultipleAlignDisplaceRMSDs_atom_major(xyz_align1, xyz_align2, g_align1, g_align2,
+                                             xyz_displ1, xyz_displ2, n_atoms_align,
+                                             n_atoms_displ, frame, parallel=True):
+    """RMSDs of ``xyz_displ1[frame]``, rotated by the rotation that best fits
+    ``xyz_align1[frame]`` onto each ``xyz_align2[i]``, from ``xyz_displ2[i]``.
+
+    Mirrors ``rmsd/_rmsd.pyx:677-759`` (arrays padded to a multiple of 4
+    atoms, the first ``n_atoms_*`` used).  Returns (rmsds float32 (F,),
+    rotations float32 (F, 3, 3), applied as x' = x R).  The alignment sets'
+    centroids are removed on the GPU (the reference expects them centred and
+    takes the traces from the caller; g_align1/g_align2 are accepted for
+    signature compatibility).
+    """
+    assert (xyz_align1.shape[2] == 3) and (xyz_align2.shape[2] == 3)
+    assert (xyz_displ1.shape[2] == 3) and (xyz_displ2.shape[2] == 3)
+    if not ((xyz_align1.shape[1] % 4 == 0) & (xyz_align2.shape[1] % 4 == 0)):
+        raise ValueError("Input arrays must have middle dimension of 4*n, "
+                         "found %d and %d." % (xyz_align1.shape[1], xyz_align2.shape[1]))
+    if not ((xyz_displ1.shape[1] % 4 == 0) & (xyz_displ2.shape[1] % 4 == 0)):
+        raise ValueError("Input arrays must have middle dimension of 4*n, "
+                         "found %d and %d." % (xyz_displ1.shape[1], xyz_displ2.shape[1]))
+    if not xyz_align1.shape[0] == xyz_displ1.shape[0]:
+        raise ValueError("xyz_align1 and xyz_displ1 must contain the same number of frames")
+    if not xyz_align2.shape[0] == xyz_displ2.shape[0]:
+        raise ValueError("xyz_align2 and xyz_displ2 must contain the same number of frames")
+    if frame >= xyz_align1.shape[0]:
+        raise ValueError("Cannot calculate RMSD of frame %d: xyz1 has "
+                         "only %d frames." % (frame, xyz_align1.shape[0]))
+    n_atoms_align, n_atoms_displ = int(n_atoms_align), int(n_atoms_displ)
+    if not (0 < n_atoms_align <= min(xyz_align1.shape[1], xyz_align2.shape[1])
+            and 0 < n_atoms_displ <= min(xyz_displ1.shape[1], xyz_displ2.shape[1])):
+        raise ValueError("n_atoms_align / n_atoms_displ must be within the arrays' atoms")
+    lib = load()
+    on_device = dev.is_device_array(xyz_align2)
+    A = dev.to_device_f32(xyz_align2)
+    D = dev.to_device_f32(xyz_displ2)
+    A_ref = dev.to_device_f32(xyz_align1[frame][:n_atoms_align]).contiguous()
+    D_ref = dev.to_device_f32(xyz_displ1[frame][:n_atoms_displ]).contiguous()
+    n_frames = int(A.shape[0])
+    out = dev.empty_f32(n_frames)
+    rot = dev.empty_f32(n_frames * 9)
+    if n_frames:
+        ws = dev.workspace(lib.hr_align_displace_workspace_bytes(n_frames, n_atoms_align))
+        call("hr_rmsd_align_displace", A.data_ptr(), n_frames, int(A.shape[1]), n_atoms_align,
+             A_ref.data_ptr(), D.data_ptr(), int(D.shape[1]), n_atoms_displ, D_ref.data_ptr(),
+             out.data_ptr(), rot.data_ptr(), ws.data_ptr(), dev.stream())
+    rot = rot.reshape(n_frames, 3, 3)
+    return _result(out, on_device), _result(rot, on_device)

@@ -76,6 +76,10 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_FRAMES_WAVES_PER_SIMD must be 0, 3 or 4 (got %d)", value);
         return HR_ERR_INVALID;
     }
+    if (knob == HR_TUNE_FRAMES_GRID && value != 0 && value != 1 && value != 2) {
+        hr::set_error("HR_TUNE_FRAMES_GRID must be 0, 1 or 2 (got %d)", value);
+        return HR_ERR_INVALID;
+    }
     hr::g_tuning[knob].store(value);
     return HR_OK;
 }

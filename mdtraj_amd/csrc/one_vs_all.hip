@@ -67,10 +67,12 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
         rs[2] = ref_stats[2];
         rs[3] = ref_stats[3];
     }
+    // whole frame quads per wave (a quad fills the wave's four rows)
     const long long wave = (long long)blockIdx.x * kWavesPerBlock + wv;
     const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
-    const long long f_begin = wave * n_frames / n_waves;
-    const long long f_end = (wave + 1) * n_frames / n_waves;
+    const long long n_quads = (n_frames + 3) / 4;
+    const long long f_begin = 4 * (wave * n_quads / n_waves);
+    const long long f_end = min(n_frames, 4 * ((wave + 1) * n_quads / n_waves));
 
     for (long long base = f_begin; base < f_end; base += kWave) {
         const int cnt = (int)min((long long)kWave, f_end - base);
@@ -185,6 +187,26 @@ int resident_blocks(Kernel kernel, size_t lds, long long n_waves_wanted, int* bl
     return HR_OK;
 }
 
+// Grid of the per-frame kernel.  The kernel gives each wave whole frame
+// quads.  With Q quads and S resident wave slots, k = ceil(Q / S) quads per
+// wave needs only ceil(Q / k) waves; launching that many gives every wave k
+// or k - 1 quads, so all waves drain together.  Filling all S slots instead
+// leaves a tail whenever Q / S lies just above an integer (100,000 frames on
+// 3,072 slots: 8.14 quads per wave on average, 9 on the slowest waves).
+// HR_TUNE_FRAMES_GRID = 2 keeps one wave per slot.
+template <typename Kernel>
+int frames_grid(Kernel kernel, size_t lds, long long n_frames, int* blocks) {
+    const long long n_quads = (n_frames + 3) / 4;
+    int rc = resident_blocks(kernel, lds, n_quads, blocks);
+    if (rc || tuning(HR_TUNE_FRAMES_GRID) == 2) return rc;
+    const long long slots = (long long)*blocks * kWavesPerBlock;
+    const long long per_wave = (n_quads + slots - 1) / slots;
+    const long long waves = (n_quads + per_wave - 1) / per_wave;
+    const long long b = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    *blocks = (int)(b < 1 ? 1 : b);
+    return HR_OK;
+}
+
 template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves>
 int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
                     long long frame_stride, int rowstride, const int* idx, int n_sel,
@@ -192,7 +214,7 @@ int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long 
                     float* out, float* rot_out, double* mu_out) {
     int blocks = 0, rc;
     auto k = frames_kernel<kKind, kAxis, kGather, kLds, kMinWaves>;
-    if ((rc = resident_blocks(k, l.lds, n_frames, &blocks))) return rc;
+    if ((rc = frames_grid(k, l.lds, n_frames, &blocks))) return rc;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
                        rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
                        mu_out);
@@ -225,6 +247,156 @@ int prep_ref(hipStream_t s, const float* ref, const int* ref_idx, int n_sel, int
     hipLaunchKernelGGL(ref_prep_kernel, dim3(1), dim3(kBlock), 0, s, ref, ref_idx, n_sel,
                        axis_rowstride, *ref_sel, *stats);
     return launch_status("ref_prep_kernel");
+}
+
+
+// Align/displace: RMS displacement of every frame's displacement set,
+// rotated by its fitting rotation R_f (x R_f, about the origin), from the
+// reference displacement set, in FP64; one wave per frame.  Also writes
+// R_f^T, the rotation taking the reference onto frame f, as the reference's
+// getMultipleAlignDisplaceRMSDs_atom_major returns it (rmsd/_rmsd.pyx:744-758
+// with rot_msd_atom_major, rmsd/src/rotation_generic.h:47-102).
+__global__ __launch_bounds__(kBlock) void rot_msd_kernel(const float* __restrict__ displ,
+                                                         long long n_frames, int displ_stride,
+                                                         int n_displ,
+                                                         const float* __restrict__ displ_ref,
+                                                         const float* __restrict__ rot,
+                                                         float* __restrict__ out,
+                                                         float* __restrict__ rot_t) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
+    for (long long f = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; f < n_frames;
+         f += n_waves) {
+        double R[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) R[i] = (double)rot[9 * f + i];
+        const float* fr = displ + f * 3LL * displ_stride;
+        double acc = 0.0;
+        for (int a = lane; a < n_displ; a += kWave) {
+            const double x = fr[3 * a], y = fr[3 * a + 1], z = fr[3 * a + 2];
+            const double dx = x * R[0] + y * R[3] + z * R[6] - (double)displ_ref[3 * a];
+            const double dy = x * R[1] + y * R[4] + z * R[7] - (double)displ_ref[3 * a + 1];
+            const double dz = x * R[2] + y * R[5] + z * R[8] - (double)displ_ref[3 * a + 2];
+            acc += dx * dx + dy * dy + dz * dz;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) out[f] = (float)sqrt(acc / n_displ);
+        if (rot_t && lane < 9) rot_t[9 * f + lane] = rot[9 * f + 3 * (lane % 3) + lane / 3];
+    }
+}
+
+// Centroid (FP64) of every frame's selection; one wave per frame.
+__global__ __launch_bounds__(kBlock) void centroid_kernel(const float* __restrict__ xyz,
+                                                          long long n_frames, int n_atoms,
+                                                          const int* __restrict__ idx, int n_sel,
+                                                          double* __restrict__ mu) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
+    for (long long f = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; f < n_frames;
+         f += n_waves) {
+        const float* fr = xyz + f * 3LL * n_atoms;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = lane; j < n_sel; j += kWave) {
+            const int a = idx ? idx[j] : j;
+            sx += fr[3 * a];
+            sy += fr[3 * a + 1];
+            sz += fr[3 * a + 2];
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        if (lane == 0) {
+            mu[3 * f] = sx / n_sel;
+            mu[3 * f + 1] = sy / n_sel;
+            mu[3 * f + 2] = sz / n_sel;
+        }
+    }
+}
+
+// RMSF, pass 1: thread = selected atom, block row = chunk of frames.  The
+// displaced coordinate of frame f is (x - mu_f) R_f, with mu_f = 0 unless
+// the selections are centred and R_f = I for frames already aligned.  Sums
+// of d = displaced - pivot (the atom's displaced coordinate in frame 0) and
+// of |d|^2 in FP64, so E|d|^2 - |E d|^2 does not cancel far from the origin.
+__global__ __launch_bounds__(kBlock) void rmsf_accum_kernel(
+    const float* __restrict__ xyz, long long n_frames, int n_atoms, const int* __restrict__ idx,
+    int n_sel, const float* __restrict__ rot, const double* __restrict__ mu, int n_chunks,
+    double* __restrict__ partial) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n_sel) return;
+    const int a = idx ? idx[j] : j;
+    const long long f0 = (long long)blockIdx.y * n_frames / n_chunks;
+    const long long f1 = (long long)(blockIdx.y + 1) * n_frames / n_chunks;
+    auto displaced = [&](long long f, double& x, double& y, double& z) {
+        const float* p = xyz + f * 3LL * n_atoms + 3LL * a;
+        double px = p[0], py = p[1], pz = p[2];
+        if (mu) {
+            px -= mu[3 * f];
+            py -= mu[3 * f + 1];
+            pz -= mu[3 * f + 2];
+        }
+        if (rot) {
+            const float* R = rot + 9 * f;
+            x = px * R[0] + py * R[3] + pz * R[6];
+            y = px * R[1] + py * R[4] + pz * R[7];
+            z = px * R[2] + py * R[5] + pz * R[8];
+        } else {
+            x = px;
+            y = py;
+            z = pz;
+        }
+    };
+    double qx, qy, qz;
+    displaced(0, qx, qy, qz);
+    double sx = 0.0, sy = 0.0, sz = 0.0, ss = 0.0;
+    for (long long f = f0; f < f1; ++f) {
+        double x, y, z;
+        displaced(f, x, y, z);
+        x -= qx;
+        y -= qy;
+        z -= qz;
+        sx += x;
+        sy += y;
+        sz += z;
+        ss += x * x + y * y + z * z;
+    }
+    double* o = partial + ((size_t)blockIdx.y * n_sel + j) * 4;
+    o[0] = sx;
+    o[1] = sy;
+    o[2] = sz;
+    o[3] = ss;
+}
+
+// RMSF, pass 2: per atom, sum the chunks; sqrt(E|d|^2 - |E d|^2).
+__global__ __launch_bounds__(kBlock) void rmsf_finish_kernel(const double* __restrict__ partial,
+                                                             int n_chunks, int n_sel,
+                                                             long long n_frames,
+                                                             float* __restrict__ out) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n_sel) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, ss = 0.0;
+    for (int c = 0; c < n_chunks; ++c) {
+        const double* p = partial + ((size_t)c * n_sel + j) * 4;
+        sx += p[0];
+        sy += p[1];
+        sz += p[2];
+        ss += p[3];
+    }
+    const double inv = 1.0 / (double)n_frames;
+    const double mx = sx * inv, my = sy * inv, mz = sz * inv;
+    const double msf = ss * inv - (mx * mx + my * my + mz * mz);
+    out[j] = (float)sqrt(msf > 0.0 ? msf : 0.0);
+}
+
+// Frame chunks of the RMSF accumulation: about 4,096 workgroups in all, at
+// most 1,024 chunks and never more chunks than frames (sizes the workspace,
+// so it depends on the problem only).
+int rmsf_chunks(long long n_frames, int n_sel) {
+    const long long atom_blocks = (n_sel + kBlock - 1) / kBlock;
+    long long c = (4096 + atom_blocks - 1) / atom_blocks;
+    if (c > 1024) c = 1024;
+    if (c > n_frames) c = n_frames;
+    return (int)(c < 1 ? 1 : c);
 }
 
 }  // namespace
@@ -415,6 +587,115 @@ int hr_superpose(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ali
                        (long long)n_frames, n_atoms, rot, mu, stats, (double)n_sel,
                        (int)(translate != 0));
     return launch_status("apply_kernel");
+}
+
+size_t hr_align_displace_workspace_bytes(int64_t n_frames, int32_t n_align) {
+    return hr_superpose_workspace_bytes(n_frames, n_align);
+}
+
+int hr_rmsd_align_displace(const float* align, int64_t n_frames, int32_t align_stride,
+                           int32_t n_align, const float* align_ref, const float* displ,
+                           int32_t displ_stride, int32_t n_displ, const float* displ_ref,
+                           float* rmsd_out, float* rot_out, void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames >= 0 && n_align > 0 && n_displ > 0 && align_stride >= n_align &&
+                   displ_stride >= n_displ,
+               "invalid sizes n_frames=%lld n_align=%d (stride %d) n_displ=%d (stride %d)",
+               (long long)n_frames, n_align, align_stride, n_displ, displ_stride);
+    if (n_frames == 0) return HR_OK;
+    HR_REQUIRE(align && align_ref && displ && displ_ref && rmsd_out && workspace,
+               "NULL pointer argument");
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = static_cast<char*>(workspace);
+    size_t off = (hr_rmsd_workspace_bytes(n_align) + 255) & ~(size_t)255;
+    double* mu = reinterpret_cast<double*>(ws + off);
+    off += (size_t)n_frames * 3 * sizeof(double);
+    off = (off + 255) & ~(size_t)255;
+    float* rot = reinterpret_cast<float*>(ws + off);
+    float* ref_sel;
+    double* stats;
+    int rc = prep_ref(s, align_ref, nullptr, n_align, 0, workspace, &ref_sel, &stats);
+    if (rc) return rc;
+    Launch l;
+    if ((rc = plan<Kind::Fit>(n_align, false, &l))) return rc;
+    rc = launch_frames<Kind::Fit, false, false>(l, s, align, n_frames, 3LL * align_stride, 0,
+                                                nullptr, n_align, ref_sel, stats, -1, nullptr,
+                                                rot, mu);
+    if (rc) return rc;
+    const DeviceInfo* d = current_device();
+    long long blocks = (long long)d->cu_count * 8;
+    const long long by_work = (n_frames + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > by_work) blocks = by_work;
+    hipLaunchKernelGGL(rot_msd_kernel, dim3((int)blocks), dim3(kBlock), 0, s, displ,
+                       (long long)n_frames, displ_stride, n_displ, displ_ref, rot, rmsd_out,
+                       rot_out);
+    return launch_status("rot_msd_kernel");
+}
+
+size_t hr_rmsf_workspace_bytes(int64_t n_frames, int32_t n_sel) {
+    size_t b = hr_superpose_workspace_bytes(n_frames, n_sel);
+    b = (b + 255) & ~(size_t)255;
+    b += (size_t)rmsf_chunks(n_frames > 0 ? n_frames : 1, n_sel > 0 ? n_sel : 1) *
+         (size_t)(n_sel > 0 ? n_sel : 0) * 4 * sizeof(double);
+    return b;
+}
+
+int hr_rmsf(const float* xyz, int64_t n_frames, int32_t n_atoms, const int32_t* atom_idx,
+            int32_t n_sel, const float* ref_frame, const int32_t* ref_idx, int32_t center,
+            float* out, void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_frames > 0 && n_atoms > 0 && n_sel > 0,
+               "invalid sizes n_frames=%lld n_atoms=%d n_sel=%d", (long long)n_frames, n_atoms,
+               n_sel);
+    HR_REQUIRE(atom_idx != nullptr || n_sel == n_atoms,
+               "n_sel (%d) must equal n_atoms (%d) without atom_idx", n_sel, n_atoms);
+    HR_REQUIRE(xyz && out && workspace, "NULL pointer argument");
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = static_cast<char*>(workspace);
+    size_t off = (hr_rmsd_workspace_bytes(n_sel) + 255) & ~(size_t)255;
+    double* mu = reinterpret_cast<double*>(ws + off);
+    off += (size_t)n_frames * 3 * sizeof(double);
+    off = (off + 255) & ~(size_t)255;
+    float* rot = reinterpret_cast<float*>(ws + off);
+    off += (size_t)n_frames * 9 * sizeof(float);
+    off = (off + 255) & ~(size_t)255;
+    double* partial = reinterpret_cast<double*>(ws + off);
+    const DeviceInfo* d = current_device();
+    if (!d) return HR_ERR_HIP;
+    int rc;
+    if (ref_frame) {
+        float* ref_sel;
+        double* stats;
+        if ((rc = prep_ref(s, ref_frame, ref_idx, n_sel, 0, workspace, &ref_sel, &stats)))
+            return rc;
+        Launch l;
+        if ((rc = plan<Kind::Fit>(n_sel, atom_idx != nullptr, &l))) return rc;
+        if (atom_idx)
+            rc = launch_frames<Kind::Fit, false, true>(l, s, xyz, n_frames, 3LL * n_atoms, 0,
+                                                       atom_idx, n_sel, ref_sel, stats, -1,
+                                                       nullptr, rot, mu);
+        else
+            rc = launch_frames<Kind::Fit, false, false>(l, s, xyz, n_frames, 3LL * n_atoms, 0,
+                                                        nullptr, n_sel, ref_sel, stats, -1,
+                                                        nullptr, rot, mu);
+        if (rc) return rc;
+    } else if (center) {
+        long long blocks = (long long)d->cu_count * 8;
+        const long long by_work = (n_frames + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (blocks > by_work) blocks = by_work;
+        hipLaunchKernelGGL(centroid_kernel, dim3((int)blocks), dim3(kBlock), 0, s, xyz,
+                           (long long)n_frames, n_atoms, atom_idx, n_sel, mu);
+        if ((rc = launch_status("centroid_kernel"))) return rc;
+    }
+    const int n_chunks = rmsf_chunks(n_frames, n_sel);
+    const int atom_blocks = (n_sel + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(rmsf_accum_kernel, dim3(atom_blocks, n_chunks), dim3(kBlock), 0, s, xyz,
+                       (long long)n_frames, n_atoms, atom_idx, n_sel, ref_frame ? rot : nullptr,
+                       center ? mu : nullptr, n_chunks, partial);
+    if ((rc = launch_status("rmsf_accum_kernel"))) return rc;
+    hipLaunchKernelGGL(rmsf_finish_kernel, dim3(atom_blocks), dim3(kBlock), 0, s, partial,
+                       n_chunks, n_sel, (long long)n_frames, out);
+    return launch_status("rmsf_finish_kernel");
 }
 
 }  // extern "C"

@@ -179,6 +179,71 @@ def superpose(xyz, ref_xyz, frame=0, atom_indices=None, ref_atom_indices=None):
     return np.einsum("fnk,fkj->fnj", x - mu, R) + ref_mu
 
 
+def rmsf(target_xyz, ref_xyz=None, frame=0, atom_indices=None, ref_atom_indices=None):
+    """FP64 ``md.rmsf`` in atom mode: per-atom RMS fluctuation (n_sel,).
+
+    Follows ``rmsd/_rmsd.pyx:246-443``.  Unless ``ref_xyz`` is None (frames
+    already aligned), every frame's selection is fitted onto the centred
+    reference selection and the fitting rotation is applied to the selection
+    (``:405-414``).  The rotated coordinates are the selection centred on its
+    centroid when ``atom_indices`` is None, and the raw selection otherwise:
+    there the reference centres a view of ``target.xyz`` in place before it
+    copies the coordinates it rotates (``:389,395-401,404``), while a fancy
+    index takes a copy first.  Fluctuation = mean over frames of |x - <x>|^2
+    (``:416-440``), then sqrt.
+    """
+    x = _select(np.asarray(target_xyz, np.float64), atom_indices)
+    mu = x.mean(axis=1, keepdims=True)
+    disp = x - mu if atom_indices is None else x.copy()
+    if ref_xyz is not None:
+        if ref_atom_indices is None:
+            ref_atom_indices = atom_indices
+        ref = np.asarray(_select(np.asarray(ref_xyz)[frame][None], ref_atom_indices),
+                         np.float64)[0]
+        R = kabsch_rotation(x - mu, ref - ref.mean(axis=0))
+        disp = np.einsum("fnk,fkj->fnj", disp, R)
+    avg = disp.mean(axis=0)
+    return np.sqrt(((disp - avg) ** 2).sum(axis=2).mean(axis=0))
+
+
+def rmsf_by_residue(rmsf_atoms, residue_of_atom, masses, n_residues):
+    """Residue mode of ``md.rmsf`` (``rmsd/_rmsd.pyx:445-484``): per residue,
+    sqrt(sum m_i f_i^2 / sum m_i) over its selected atoms, f_i the per-atom
+    RMSF (the reference combines the mean-square fluctuations, before any
+    square root); -1 for residues without selected atoms."""
+    num = np.zeros(n_residues)
+    den = np.zeros(n_residues)
+    msf = np.asarray(rmsf_atoms, np.float64) ** 2
+    np.add.at(num, residue_of_atom, msf * masses)
+    np.add.at(den, residue_of_atom, masses)
+    out = np.full(n_residues, -1.0)
+    ok = den > 0
+    out[ok] = np.sqrt(num[ok] / den[ok])
+    return out
+
+
+def align_displace_rmsd(align1, align2, displ1, displ2, frame):
+    """FP64 ``getMultipleAlignDisplaceRMSDs_atom_major`` (``rmsd/_rmsd.pyx:677-759``):
+    the rotation R_i that best fits ``align1[frame]`` onto ``align2[i]``
+    (``msd_atom_major(align1[frame], align2[i])``, applied as x R), then the
+    RMS displacement of ``displ1[frame] R_i`` from ``displ2[i]``
+    (``rot_msd_atom_major``, ``rmsd/src/rotation_generic.h:47-102``), with no
+    centering of the displacement sets.  The alignment sets are centred here
+    (the reference requires the caller to have centred them).  Returns
+    (rmsd (F,), R (F, 3, 3))."""
+    a1 = np.asarray(align1, np.float64)[frame]
+    a2 = np.asarray(align2, np.float64)
+    ref = a1 - a1.mean(axis=0)
+    mob = a2 - a2.mean(axis=1, keepdims=True)
+    # optimal rotation of mobile frame i onto the reference, transposed:
+    # the rotation taking the reference onto frame i
+    R = np.transpose(kabsch_rotation(mob, ref), (0, 2, 1))
+    d1 = np.asarray(displ1, np.float64)[frame]
+    d2 = np.asarray(displ2, np.float64)
+    moved = np.einsum("nk,fkj->fnj", d1, R)
+    return np.sqrt(((moved - d2) ** 2).sum(axis=2).mean(axis=1)), R
+
+
 def rmsd_matrix(xyz, atom_indices=None):
     """All-pairs RMSD matrix D[i, j] in FP64 (F, F).
 
