@@ -90,7 +90,36 @@ def cpu_baseline_one_vs_all(X, idx, budget_s=12.0, sample_frames=10_000):
         "sample": f"first {host.shape[0]} frames of the same trajectory, same 1,500-atom "
                   f"selection, {reps} repeats ({t_total:.1f} s); numpy gather + centering + "
                   "QCP per frame, OpenMP over frames",
+        "small_case": small_case_cpu_and_gpu(impl),
     }, first
+
+
+def small_case_cpu_and_gpu(impl, repeats=200):
+    """The reference's own CPU-sized case: one reference vs 1,000 frames x 22
+    atoms (seeded cube, noise 0.02 nm, random rotation and translation).
+    Median frame-pairs/s of the CPU core and of md.rmsd through the GPU path
+    (host arrays in and out, so launch and copy costs included)."""
+    import numpy as np
+
+    import mdtraj_amd as md
+    from mdtraj_amd import gen
+
+    x, _ = gen.small_cube(seed=0, n_frames=1000, n_atoms=22, sigma=0.02)
+    out = np.zeros(x.shape[0], np.float32)
+    cpu, gpu = [], []
+    t = md.Trajectory(x)
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        target = np.array(x, order="C", dtype=np.float32)
+        impl.rmsd_inplace(target, np.array(x[0]), out, True)
+        cpu.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        md.rmsd(t, t, 0)
+        gpu.append(time.perf_counter() - t0)
+    return {"workload": "1 reference vs 1,000 frames x 22 atoms (host arrays)",
+            "cpu_frame_pairs_per_s": 1000 / float(np.median(cpu)),
+            "gpu_md_rmsd_frame_pairs_per_s": 1000 / float(np.median(gpu)),
+            "repeats": repeats}
 
 
 def bench_one_vs_all(args):
