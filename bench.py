@@ -309,8 +309,9 @@ def bench_batched(args):
     xs = gen.uneven_systems_device(seed=3)
     torch.cuda.synchronize()
     best, table = autotune.tune_batch(xs, reps=3)
-    log("autotuner: " + ", ".join(f"fpu={k[0]} waves={k[1]}: {v:.3f} ms" for k, v in table.items()))
-    b = BatchRMSD(xs, frames_per_unit=best[0], waves=best[1])
+    log("autotuner: " + ", ".join(f"fpu={k[0]} kib={k[1]} waves={k[2]}: {v:.3f} ms"
+                                  for k, v in table.items()))
+    b = BatchRMSD(xs, frames_per_unit=best[0], unit_kib=best[1], waves=best[2])
     for _ in range(args.warmup):
         b.run()
     torch.cuda.synchronize()
@@ -334,7 +335,7 @@ def bench_batched(args):
         "config": {"workload": "512 systems, atoms log-uniform [50, 5000], frames log-uniform "
                                "[10, 2000], sigma 0.03 nm, vs own frame 0",
                    "total_frames": b.total_frames, "total_bytes": b.total_bytes,
-                   "units": b.n_units, "frames_per_unit": best[0], "waves_per_simd": best[1],
+                   "units": b.n_units, "frames_per_unit": best[0], "unit_kib": best[1], "waves_per_simd": best[2],
                    "graph_ms": ms},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "traffic": None},
@@ -383,6 +384,7 @@ def bench_serving(args):
     t0 = time.perf_counter()
     t_next = t0
     reqs = []
+    t_log = t0
     for k in range(n_req):
         t_next += gaps[k]
         while True:
@@ -391,8 +393,15 @@ def bench_serving(args):
                 break
             time.sleep(min(t_next - now, 0.0005))
         reqs.append(srv.submit(int(pools[k]), samples[pools[k]][: qs[k]], t_arrival=t_next))
-    for r in reqs:
+        if now - t_log >= 1.0:
+            t_log = now
+            log(f"serving: {k + 1} requests submitted, {len(srv.completed)} answered, "
+                f"{srv.batches} batches")
+    for i, r in enumerate(reqs):
         r.wait(600)
+        if time.perf_counter() - t_log >= 1.0:
+            t_log = time.perf_counter()
+            log(f"serving: draining, {i + 1} of {n_req} answered")
     t1 = max(r.t_done for r in reqs)
     srv.stop()
     lat = np.array([r.latency for r in reqs]) * 1e3
@@ -483,8 +492,8 @@ def main():
     ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
                     help="register budget of the per-frame kernel (0 = library default)")
     ap.add_argument("--frames-grid", type=int, default=0, choices=[0, 1, 2],
-                    help="work split of the per-frame kernel: 1 balanced quads, 2 one wave "
-                         "per resident slot (0 = library default)")
+                    help="grid of the per-frame kernel: 1 equal quad counts, 2 one wave "
+                         "per resident slot (0 = library default, 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "

@@ -52,11 +52,11 @@ int hr_device_cu_count(void);
  * mdtraj_amd/autotune.py).  Values are process-wide; 0 restores the default.
  *   HR_TUNE_FRAMES_WAVES_PER_SIMD  register budget of the per-frame kernel:
  *                                  3 (default, no spills) or 4 waves/SIMD
- *   HR_TUNE_FRAMES_GRID            work split of the per-frame kernel:
- *                                  1 (default) whole frame quads, as many
- *                                  waves as give every wave the same count;
- *                                  2 every resident wave slot, frames split
- *                                  evenly */
+ *   HR_TUNE_FRAMES_GRID            grid of the per-frame kernel (whole
+ *                                  frame quads per wave either way):
+ *                                  2 (default) one wave per resident slot;
+ *                                  1 only as many waves as give every wave
+ *                                  the same quad count */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_COUNT 2
@@ -249,9 +249,19 @@ int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t
  * launch.  Plan creation allocates and copies the descriptor and unit tables;
  * launches allocate nothing.  The systems' buffers must stay valid for the
  * plan's lifetime.  Do not launch one plan on two streams at once.
- *   frames_per_unit  frames per work unit, 1..64 (0 = 64); the autotuner in
- *                    mdtraj_amd/autotune.py picks it per workload
+ *   shape            work-unit shape (NULL = defaults); the autotuner in
+ *                    mdtraj_amd/autotune.py picks it per workload:
+ *     frames_per_unit  frames per work unit, 1..64 (0 = 64)
+ *     unit_kib         if > 0, a system's units hold fewer frames (a multiple
+ *                      of 4, at least 4) when frames_per_unit frames would
+ *                      exceed unit_kib KiB of coordinates
+ *     waves_per_simd   register budget of the scoring kernel: 3 (0 = 3) or 4
  * ------------------------------------------------------------------------- */
+typedef struct hr_batch_shape {
+    int32_t frames_per_unit;
+    int32_t unit_kib;
+    int32_t waves_per_simd;
+} hr_batch_shape;
 typedef struct hr_system_desc {
     const float* xyz;  /* (n_frames, n_atoms, 3) float32, device */
     int64_t n_frames;
@@ -261,7 +271,7 @@ typedef struct hr_system_desc {
 } hr_system_desc;
 typedef struct hr_batch_plan hr_batch_plan;
 int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
-                         int32_t frames_per_unit, hr_batch_plan** plan);
+                         const hr_batch_shape* shape, hr_batch_plan** plan);
 int hr_batch_plan_launch(hr_batch_plan* plan, void* stream);
 int hr_batch_plan_destroy(hr_batch_plan* plan);
 int64_t hr_batch_plan_units(const hr_batch_plan* plan);

@@ -54,7 +54,7 @@ _SIGNATURES = {
     "hr_split_bytes": (_sz, [_i64, _i32, _i32]),
     "hr_split_prepare": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _sz, _vp]),
     "hr_rmsd_cross": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _vp]),
-    "hr_batch_plan_create": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "hr_batch_plan_create": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "hr_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
     "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "hr_batch_plan_units": (_i64, [_vp]),
@@ -70,6 +70,12 @@ class SystemDesc(ctypes.Structure):
     _fields_ = [("xyz", ctypes.c_void_p), ("n_frames", ctypes.c_int64),
                 ("n_atoms", ctypes.c_int32), ("ref_frame", ctypes.c_int32),
                 ("out", ctypes.c_void_p)]
+
+
+class BatchShape(ctypes.Structure):
+    """hr_batch_shape of include/hiprmsd.h."""
+    _fields_ = [("frames_per_unit", ctypes.c_int32), ("unit_kib", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32)]
 
 
 class HipRmsdError(RuntimeError):

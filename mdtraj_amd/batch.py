@@ -11,20 +11,24 @@ from __future__ import annotations
 import ctypes
 
 from . import _device as dev
-from ._lib import HR_TUNE_FRAMES_WAVES_PER_SIMD, SystemDesc, call, load
+from ._lib import BatchShape, SystemDesc, call, load
 
 
 class BatchRMSD:
     """RMSD of every frame of each system to that system's ``ref_frames[s]``.
 
     ``systems``: list of float32 (F_s, N_s, 3) arrays or CUDA tensors (moved
-    to the device once).  ``frames_per_unit``: work-unit size in frames
-    (1..64) and ``waves`` (3 or 4 waves per SIMD of the scoring kernel); None
-    picks the autotuned value for this batch (``mdtraj_amd.autotune``).  ``run()`` is one graph launch on torch's
-    current stream and returns the list of float32 CUDA tensors of RMSDs.
+    to the device once).  Work-unit shape: ``frames_per_unit`` (1..64),
+    ``unit_kib`` (if > 0, fewer frames per unit, a multiple of 4, where
+    frames_per_unit frames would exceed that many KiB) and ``waves`` (3 or 4
+    waves per SIMD of the scoring kernel); None picks the autotuned value for
+    this batch (``mdtraj_amd.autotune``).  ``run()`` is one graph launch on
+    torch's current stream and returns the list of float32 CUDA tensors of
+    RMSDs.
     """
 
-    def __init__(self, systems, ref_frames=None, frames_per_unit=None, waves=None):
+    def __init__(self, systems, ref_frames=None, frames_per_unit=None, waves=None,
+                 unit_kib=None):
         load()
         self.xyz = [dev.to_device_f32(x) for x in systems]
         for i, x in enumerate(self.xyz):
@@ -35,13 +39,15 @@ class BatchRMSD:
             raise ValueError("no systems")
         self.ref_frames = [0] * n if ref_frames is None else [int(r) for r in ref_frames]
         self.out = [dev.empty_f32(max(int(x.shape[0]), 1)) for x in self.xyz]
-        if frames_per_unit is None or waves is None:
+        if frames_per_unit is None or waves is None or unit_kib is None:
             from .autotune import tuned_shape
 
-            t_fpu, t_waves = tuned_shape(self.xyz, self.ref_frames)
+            t_fpu, t_kib, t_waves = tuned_shape(self.xyz, self.ref_frames)
             frames_per_unit = t_fpu if frames_per_unit is None else frames_per_unit
+            unit_kib = t_kib if unit_kib is None else unit_kib
             waves = t_waves if waves is None else waves
         self.frames_per_unit = int(frames_per_unit)
+        self.unit_kib = int(unit_kib)
         self.waves = int(waves)
         self._plan = None
         self._build()
@@ -54,16 +60,11 @@ class BatchRMSD:
         return arr
 
     def _build(self):
-        lib = load()
         plan = ctypes.c_void_p()
         descs = self._descs()
-        prev = lib.hr_get_tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD)
-        call("hr_set_tuning", HR_TUNE_FRAMES_WAVES_PER_SIMD, self.waves)
-        try:
-            call("hr_batch_plan_create", ctypes.addressof(descs), len(self.xyz),
-                 self.frames_per_unit, ctypes.byref(plan))
-        finally:
-            lib.hr_set_tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD, prev)
+        shape = BatchShape(self.frames_per_unit, self.unit_kib, self.waves)
+        call("hr_batch_plan_create", ctypes.addressof(descs), len(self.xyz),
+             ctypes.addressof(shape), ctypes.byref(plan))
         self._plan = plan
 
     @property

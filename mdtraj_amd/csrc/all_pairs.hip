@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half
                 }
                 const double msd = qcp_msd(M, g.traces_a[i], g.traces_b[j], n);
                 const long long o = kRect ? i * g.out_ld + j : condensed_index(i, j, g.n_a);
-                g.out[o] = (float)sqrt(msd);
+                g.out[o] = sqrtf((float)msd);  // one float rounding, then the float root
             }
         }
         __syncthreads();

@@ -76,11 +76,13 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     double* s_res = smem_d + wv * kWave * kNS;
-    for (;;) {
-        unsigned u = 0;
-        if (lane == 0) u = atomicAdd(ticket, 1u);
-        u = __shfl(u, 0, kWave);
-        if ((long long)u >= n_units) break;
+    unsigned u = 0;
+    if (lane == 0) u = atomicAdd(ticket, 1u);
+    u = __shfl(u, 0, kWave);
+    while ((long long)u < n_units) {
+        // take the next ticket now; its latency hides behind this unit
+        unsigned next = 0;
+        if (lane == 0) next = atomicAdd(ticket, 1u);
         const DevUnit unit = units[u];
         const DevSystem s = sys[unit.system];
         const int n = s.n_atoms;
@@ -88,6 +90,7 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
         process_group<Kind::Rmsd, false, false>(
             s.xyz, unit.base, unit.cnt, 3LL * n, 0, nullptr, n, s.ref_sel, s.ref_sel + n,
             s.ref_sel + 2 * n, rs, (long long)s.ref_frame, s.out, nullptr, nullptr, s_res, lane);
+        u = __shfl(next, 0, kWave);
     }
 }
 
@@ -148,14 +151,20 @@ using namespace hr;
 extern "C" {
 
 int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
-                         int32_t frames_per_unit, hr_batch_plan** plan_out) {
+                         const hr_batch_shape* shape, hr_batch_plan** plan_out) {
     clear_error();
     HR_REQUIRE(plan_out != nullptr, "plan_out is NULL");
     *plan_out = nullptr;
     HR_REQUIRE(systems != nullptr && n_systems > 0, "no systems");
+    int frames_per_unit = shape ? shape->frames_per_unit : 0;
+    const int unit_kib = shape ? shape->unit_kib : 0;
+    int waves = shape ? shape->waves_per_simd : 0;
     if (frames_per_unit == 0) frames_per_unit = 64;
+    if (waves == 0) waves = 3;
     HR_REQUIRE(frames_per_unit >= 1 && frames_per_unit <= kWave,
                "frames_per_unit must be in [1, 64] (got %d)", frames_per_unit);
+    HR_REQUIRE(unit_kib >= 0, "unit_kib must be >= 0 (got %d)", unit_kib);
+    HR_REQUIRE(waves == 3 || waves == 4, "waves_per_simd must be 3 or 4 (got %d)", waves);
     for (int i = 0; i < n_systems; ++i) {
         const hr_system_desc& s = systems[i];
         HR_REQUIRE(s.n_frames >= 0 && s.n_atoms > 0, "system %d: invalid sizes", i);
@@ -181,8 +190,15 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
     // work units, largest first (LPT order keeps the tail short)
     std::vector<DevUnit> units;
     for (int i = 0; i < n_systems; ++i) {
-        for (long long b = 0; b < systems[i].n_frames; b += frames_per_unit) {
-            const long long c = std::min<long long>(frames_per_unit, systems[i].n_frames - b);
+        int fpu = frames_per_unit;
+        if (unit_kib > 0) {
+            // frames of 12 N bytes that fit the budget, as a multiple of 4 (>= 4)
+            const long long fit = (long long)unit_kib * 1024 / (12LL * systems[i].n_atoms);
+            const int cap = (int)std::max<long long>(4, fit / 4 * 4);
+            fpu = std::min(fpu, cap);
+        }
+        for (long long b = 0; b < systems[i].n_frames; b += fpu) {
+            const long long c = std::min<long long>(fpu, systems[i].n_frames - b);
             units.push_back(DevUnit{i, (int)c, b});
         }
         p->total_frames += systems[i].n_frames;
@@ -225,7 +241,7 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
                               hipMemcpyHostToDevice));
     HR_PLAN_TRY(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
 #undef HR_PLAN_TRY
-    rc = tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4 ? capture_plan<4>(p) : capture_plan<3>(p);
+    rc = waves == 4 ? capture_plan<4>(p) : capture_plan<3>(p);
     if (rc) {
         release(p);
         return rc;

@@ -109,6 +109,19 @@ __device__ __forceinline__ double seg_sum(double v, int width) {
 // (near-)double roots -- the two largest eigenvalues of K (nearly) coincide,
 // as for a centred two-atom (collinear) structure -- where P's values
 // determine the root only to about sqrt(eps) lambda.
+// 1/d.  On the GPU: the hardware estimate (v_rcp_f64) refined by one
+// Newton-Raphson step, r + r (1 - d r), in place of the IEEE division
+// sequence; the Newton iteration below corrects any last-ulp difference.
+// The host build of this code (used by the CPU tests) divides.
+__host__ __device__ __forceinline__ double recip(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(d);
+    return fma(r, fma(-d, r, 1.0), r);
+#else
+    return 1.0 / d;
+#endif
+}
+
 __host__ __device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
                                                    double m_frob, bool* resolved) {
     double x = e0, p, d1, d2;
@@ -128,7 +141,7 @@ __host__ __device__ inline double qcp_largest_root(double c2, double c1, double 
         }
     }
     for (int it = 0; it < 64; ++it) {
-        const double nx = x - p / d1;
+        const double nx = x - p * recip(d1);
         if (!(nx < x)) break;
         double np, nd1, nd2;
         eval(nx, np, nd1, nd2);
@@ -143,10 +156,12 @@ __host__ __device__ inline double qcp_largest_root(double c2, double c1, double 
     const double x2 = x * x;
     // Horner rounding bound of P at x: 4 eps times the sum of |terms|
     const double perr = 8.9e-16 * (x2 * x2 + fabs(c2) * x2 + fabs(c1) * x + fabs(c0));
+    // bracket width 8 P'' q^2 / P'^3 + 2 perr / P' (q = P + perr) against 1e-11 x,
+    // multiplied through by P'^3; an underflowing P'^3 counts as unresolved
     const double q = p + perr;
-    const double width = 8.0 * d2 * q * q / (d1 * d1 * d1) + 2.0 * perr / d1;
-    *resolved = width <= 1e-11 * x;
-    return x - p / d1;
+    const double d13 = d1 * d1 * d1;
+    *resolved = d13 > 0.0 && 8.0 * d2 * q * q + 2.0 * perr * d1 * d1 <= 1e-11 * x * d13;
+    return x - p * recip(d1);
 }
 
 // Key-matrix entries of Horn/Theobald for M[r*3+c] = sum a_r b_c, in the

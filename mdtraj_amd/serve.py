@@ -5,6 +5,9 @@ Pools are uploaded once and kept in HBM in prepared split form
 64 query frames with the same atom count as its pool and asks for the RMSD of
 every query frame to every pool frame.
 
+A failure in the scheduler or completion thread (for example a failed
+allocation) fails every outstanding request: ``Request.wait`` raises it.
+
 Continuous batching: a scheduler thread takes the oldest pending request,
 gathers every other pending request for the same pool up to
 ``max_batch_queries`` query frames, and launches the batch (upload, split
@@ -31,7 +34,7 @@ from .allpairs import MODES, SplitSet, rmsd_cross
 
 
 class Request:
-    __slots__ = ("pool", "queries", "t_arrival", "t_done", "result", "_done")
+    __slots__ = ("pool", "queries", "t_arrival", "t_done", "result", "error", "_done")
 
     def __init__(self, pool, queries, t_arrival):
         self.pool = pool
@@ -39,12 +42,19 @@ class Request:
         self.t_arrival = t_arrival
         self.t_done = None
         self.result = None
+        self.error = None
         self._done = threading.Event()
 
     def wait(self, timeout=None):
         if not self._done.wait(timeout):
             raise TimeoutError("request not finished")
+        if self.error is not None:
+            raise RuntimeError("RMSD server failed") from self.error
         return self.result
+
+    def _fail(self, exc):
+        self.error = exc
+        self._done.set()
 
     @property
     def latency(self):
@@ -73,6 +83,10 @@ class RMSDServer:
         self.inflight_bytes = 0
         self.streams = [self.t.cuda.Stream() for _ in range(n_streams)]
         self.free_streams = collections.deque(range(n_streams))
+        # one pinned result buffer per stream slot, reused by every batch on
+        # that slot (the slot is free again only after its rows were copied out)
+        self.pinned = [None] * n_streams
+        self.error = None
         self.pending = collections.deque()
         self.cv = threading.Condition()
         self.completions = collections.deque()
@@ -96,6 +110,8 @@ class RMSDServer:
 
     # -------------------------------------------------------------- requests
     def submit(self, pool_id, queries, t_arrival=None):
+        if self.error is not None:
+            raise RuntimeError("RMSD server failed") from self.error
         q = np.ascontiguousarray(queries, dtype=np.float32)
         if q.ndim == 2:
             q = q[None]
@@ -134,70 +150,107 @@ class RMSDServer:
         return pool, batch, nq
 
     def _scheduler(self):
-        while True:
-            with self.cv:
-                while self.running and (not self.pending or not self.free_streams):
-                    self.cv.wait()
-                if not self.running and not self.pending:
-                    return
-                if not self.free_streams:
-                    continue
-                pool = self.pools[self.pending[0].pool]
-                need = self._batch_bytes(pool, self.max_q)
-                while self.running and self.resident + self.inflight_bytes + need > self.budget:
-                    self.cv.wait()
-                pool, batch, nq = self._take_batch()
-                si = self.free_streams.popleft()
-                self.inflight_bytes += need
-            self._launch(pool, batch, nq, si, need)
+        try:
+            while True:
+                with self.cv:
+                    while self.running and (not self.pending or not self.free_streams):
+                        self.cv.wait()
+                    if not self.running and not self.pending:
+                        return
+                    if not self.free_streams:
+                        continue
+                    pool = self.pools[self.pending[0].pool]
+                    need = self._batch_bytes(pool, self.max_q)
+                    while (self.running
+                           and self.resident + self.inflight_bytes + need > self.budget):
+                        self.cv.wait()
+                    pool, batch, nq = self._take_batch()
+                    si = self.free_streams.popleft()
+                    self.inflight_bytes += need
+                try:
+                    self._launch(pool, batch, nq, si, need)
+                except BaseException:
+                    for r in batch:
+                        r._fail(RuntimeError("batch launch failed"))
+                    raise
+        except BaseException as exc:  # noqa: BLE001 - handed to every waiter
+            self._fail_all(exc)
+
+    def _pinned_rows(self, si, nq, nf):
+        """(nq, nf) view of stream slot si's pinned result buffer."""
+        t = self.t
+        need = nq * nf
+        buf = self.pinned[si]
+        if buf is None or buf.numel() < need:
+            buf = t.empty(max(need, self.max_q * nf), dtype=t.float32, pin_memory=True)
+            self.pinned[si] = buf
+        return buf[:need].view(nq, nf)
 
     def _launch(self, pool, batch, nq, si, need):
         t = self.t
         stream = self.streams[si]
         queries = np.concatenate([r.queries for r in batch], axis=0)
         with t.cuda.stream(stream):
-            host_in = t.from_numpy(queries).pin_memory()
-            dq = host_in.to(device=dev.device(), non_blocking=True)
+            dq = t.from_numpy(queries).to(device=dev.device())
             qs = SplitSet(dq, mode=self.mode)
             out = rmsd_cross(qs, pool.split)
-            host_out = t.empty(out.shape, dtype=t.float32, pin_memory=True)
+            host_out = self._pinned_rows(si, nq, pool.n_frames)
             host_out.copy_(out, non_blocking=True)
             ev = t.cuda.Event()
             ev.record(stream)
         with self.done_cv:
             # keep device buffers alive until the event has passed
-            self.completions.append((ev, batch, host_out, (host_in, dq, qs, out), si, need, nq,
+            self.completions.append((ev, batch, host_out, (dq, qs, out), si, need, nq,
                                      pool.n_frames))
             self.done_cv.notify_all()
 
     def _completer(self):
-        while True:
-            with self.done_cv:
-                while not self.completions and (self.running or self._sched.is_alive()):
-                    self.done_cv.wait(timeout=0.05)
-                if not self.completions:
-                    if not self.running and not self._sched.is_alive():
-                        return
-                    continue
-                ev, batch, host_out, keep, si, need, nq, nf = self.completions.popleft()
-            ev.synchronize()
-            now = time.perf_counter()
-            res = host_out.numpy()
-            row = 0
-            for r in batch:
-                q = r.queries.shape[0]
-                r.result = res[row:row + q]
-                row += q
-                r.t_done = now
-                r._done.set()
-            del keep
-            with self.cv:
-                self.free_streams.append(si)
-                self.inflight_bytes -= need
-                self.pairs_done += nq * nf
-                self.batches += 1
-                self.completed.extend(batch)
-                self.cv.notify_all()
+        try:
+            while True:
+                with self.done_cv:
+                    while not self.completions and (self.running or self._sched.is_alive()):
+                        self.done_cv.wait(timeout=0.05)
+                    if not self.completions:
+                        if not self.running and not self._sched.is_alive():
+                            return
+                        continue
+                    ev, batch, host_out, keep, si, need, nq, nf = self.completions.popleft()
+                ev.synchronize()
+                now = time.perf_counter()
+                res = host_out.numpy()
+                row = 0
+                for r in batch:
+                    q = r.queries.shape[0]
+                    r.result = res[row:row + q].copy()   # the pinned buffer is reused
+                    row += q
+                    r.t_done = now
+                    r._done.set()
+                del keep
+                with self.cv:
+                    self.free_streams.append(si)
+                    self.inflight_bytes -= need
+                    self.pairs_done += nq * nf
+                    self.batches += 1
+                    self.completed.extend(batch)
+                    self.cv.notify_all()
+        except BaseException as exc:  # noqa: BLE001 - handed to every waiter
+            self._fail_all(exc)
+
+    def _fail_all(self, exc):
+        """Stop serving and fail every request not yet answered."""
+        with self.cv:
+            self.error = exc
+            self.running = False
+            pending = list(self.pending)
+            self.pending.clear()
+            self.cv.notify_all()
+        with self.done_cv:
+            inflight = [r for c in self.completions for r in c[1]]
+            self.completions.clear()
+            self.done_cv.notify_all()
+        for r in pending + inflight:
+            if not r._done.is_set():
+                r._fail(exc)
 
     # ------------------------------------------------------------- lifecycle
     def start(self):

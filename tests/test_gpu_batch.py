@@ -22,11 +22,12 @@ def systems(seed=0, n=24):
     return out
 
 
-@pytest.mark.parametrize("fpu,waves", [(64, 3), (16, 3), (7, 4), (64, 4)])
-def test_batch_vs_fp64(gpu, fpu, waves):
+@pytest.mark.parametrize("fpu,kib,waves", [(64, 0, 3), (16, 0, 3), (7, 0, 4), (64, 0, 4),
+                                           (64, 8, 3), (8, 64, 3)])
+def test_batch_vs_fp64(gpu, fpu, kib, waves):
     xs = systems()
     refs = [int(np.random.default_rng(i).integers(0, x.shape[0])) for i, x in enumerate(xs)]
-    b = BatchRMSD(xs, refs, frames_per_unit=fpu, waves=waves)
+    b = BatchRMSD(xs, refs, frames_per_unit=fpu, unit_kib=kib, waves=waves)
     res = [r.cpu().numpy() for r in b.run()]
     for x, r, got in zip(xs, refs, res):
         want = fp64.rmsd(x, x, r)
@@ -39,7 +40,7 @@ def test_batch_vs_fp64(gpu, fpu, waves):
 def test_batch_graph_replays_are_identical(gpu):
     torch = gpu
     xs = systems(seed=3, n=16)
-    b = BatchRMSD(xs, frames_per_unit=32, waves=3)
+    b = BatchRMSD(xs, frames_per_unit=32, unit_kib=0, waves=3)
     first = [r.clone() for r in b.run()]
     torch.cuda.synchronize()
     for k in range(4):
@@ -59,9 +60,9 @@ def test_autotuner_candidates_agree(gpu):
     from mdtraj_amd import autotune
 
     xs = systems(seed=5, n=12)
-    best, results = autotune.tune_batch(xs, candidates=[(16, 3), (64, 4)], reps=2)
+    best, results = autotune.tune_batch(xs, candidates=[(16, 0, 3), (64, 32, 4)], reps=2)
     assert best in results and all(v > 0 for v in results.values())
-    a = BatchRMSD(xs, frames_per_unit=16, waves=3).run()
-    b = BatchRMSD(xs, frames_per_unit=64, waves=4).run()
+    a = BatchRMSD(xs, frames_per_unit=16, unit_kib=0, waves=3).run()
+    b = BatchRMSD(xs, frames_per_unit=64, unit_kib=32, waves=4).run()
     for u, v in zip(a, b):
         assert gpu.equal(u, v)

@@ -52,3 +52,23 @@ def test_serve_budget(gpu):
     srv = RMSDServer(hbm_budget=1e6)
     with pytest.raises(MemoryError):
         srv.add_pool(gen.walk(seed=1, n_frames=500, n_atoms=300))
+
+
+def test_serve_failure_reaches_every_waiter(gpu, monkeypatch):
+    """A failing batch (forced from the host: the cross-Gram call raises)
+    must fail the waiting requests, not leave them hanging."""
+    from mdtraj_amd import serve
+
+    def boom(*a, **k):
+        raise MemoryError("forced failure")
+
+    pool = gen.walk(seed=3, n_frames=50, n_atoms=20)
+    with RMSDServer(hbm_budget=1e9) as srv:
+        pid = srv.add_pool(pool)
+        monkeypatch.setattr(serve, "rmsd_cross", boom)
+        reqs = [srv.submit(pid, pool[:2]) for _ in range(5)]
+        for r in reqs:
+            with pytest.raises(RuntimeError):
+                r.wait(30)
+        with pytest.raises(RuntimeError):
+            srv.submit(pid, pool[:1])
