@@ -290,30 +290,48 @@ struct GramArgs {
     long long out_ld;    // rectangle: row stride of out
 };
 
+constexpr int kMaxLoads = 7;  // loads per wave per stage (waves 0-1: 7, the rest: 6)
+
+// Chunk-0 source of each of this lane's stage loads.  The split layout keeps
+// a frame's chunks 6 * kChunk halves (384 bytes) apart, so the load for chunk
+// c reads src[k] + c * 384 bytes: the per-tile address arithmetic runs once,
+// not once per chunk.  Entries k >= the wave's load count are never issued.
 template <typename Half>
-__device__ __forceinline__ void issue_stage(const GramArgs<Half>& g, long long i0, long long j0,
-                                            int n_chunks, int c, char* stage, int w, int lane) {
-    const int first = loads_first(w), count = loads_count(w);
-    for (int k = 0; k < count; ++k) {  // wave-uniform trip count
-        const int ins = first + k;
-        const int slot = ins * kWave + lane;  // < 3200
-        const int tile = slot / (kMTile * kSlots);
+__device__ __forceinline__ void plan_loads(const GramArgs<Half>& g, long long i0, long long j0,
+                                           int n_chunks, int first, int lane,
+                                           const char* (&src)[kMaxLoads]) {
+#pragma unroll
+    for (int k = 0; k < kMaxLoads; ++k) {
+        const int slot = (first + k) * kWave + lane;
+        const int tile = slot >= kMTile * kSlots;  // 0: A rows, 1: B columns
         const int sl = slot - tile * (kMTile * kSlots);
-        const int fr = sl / kSlots;
+        const int fr = min(sl / kSlots, kMTile - 1);  // slots past 2 tiles: never issued
         int r = sl - fr * kSlots;
-        if (r == kSlots - 1) r = 0;  // pad slot: harmless duplicate load
+        if (r >= kSlots - 1) r = 0;  // pad slot: harmless duplicate load
         const long long nf = tile ? g.n_b : g.n_a;
         long long f = (tile ? j0 : i0) + fr;
         if (f >= nf) f = nf - 1;     // rows past the end: never written out
         const Half* base = tile ? g.split_b : g.split_a;
-        const Half* src = base + ((f * n_chunks + c) * 6) * kChunk + r * 8;
-        __builtin_amdgcn_global_load_lds(
-            reinterpret_cast<const void*>(src),
-            (__attribute__((address_space(3))) void*)(stage + ins * kWave * 16), 16, 0, 0);
+        src[k] = reinterpret_cast<const char*>(base + (f * n_chunks * 6) * kChunk + r * 8);
     }
 }
 
-// Wait until this wave has at most one stage of loads outstanding (or none).
+// DMA of chunk c into one LDS stage: 16 bytes per lane per load, straight
+// from global memory to LDS (no staging registers).  count is wave-uniform.
+template <typename Half>
+__device__ __forceinline__ void issue_stage(const char* const (&src)[kMaxLoads], int first,
+                                            int count, int c, char* stage) {
+    const long long adv = (long long)c * 6 * kChunk * (long long)sizeof(Half);
+#pragma unroll
+    for (int k = 0; k < kMaxLoads; ++k) {
+        if (k < count)
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const void*>(src[k] + adv),
+                (__attribute__((address_space(3))) void*)(stage + (first + k) * kWave * 16), 16,
+                0, 0);
+    }
+}
+
 __device__ __forceinline__ void wait_loads(int w, bool keep_one) {
     // s_waitcnt encoding (gfx9): vmcnt[3:0] | expcnt 6:4 | lgkmcnt 11:8 | vmcnt[5:4] 15:14
     if (!keep_one) __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -383,15 +401,21 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half
         aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
     const int boff = kTileBytes + (16 * wj + (lane & 15)) * kSlots * 16 + quarter * 16;
 
-    issue_stage<Half>(g, i0, j0, n_chunks, 0, lds, w, lane);
-    if (n_chunks > 1) issue_stage<Half>(g, i0, j0, n_chunks, 1, lds + kStageBytes, w, lane);
+    // w is wave-uniform; readfirstlane lets the compiler keep the load
+    // count and the LDS offsets in scalar registers
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int first = loads_first(wu), count = loads_count(wu);
+    const char* src[kMaxLoads];
+    plan_loads<Half>(g, i0, j0, n_chunks, first, lane, src);
+    issue_stage<Half>(src, first, count, 0, lds);
+    if (n_chunks > 1) issue_stage<Half>(src, first, count, 1, lds + kStageBytes);
     for (int c = 0; c < n_chunks; ++c) {
         wait_loads(w, c + 1 < n_chunks);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
         __builtin_amdgcn_s_barrier();        // every wave's chunk-c loads have landed
         if (c + 2 < n_chunks)
-            issue_stage<Half>(g, i0, j0, n_chunks, c + 2, lds + ((c + 2) % kStages) * kStageBytes,
-                              w, lane);
+            issue_stage<Half>(src, first, count, c + 2,
+                              lds + ((c + 2) % kStages) * kStageBytes);
         if (!active) continue;
         const char* st = lds + (c % kStages) * kStageBytes;
         vec b[6];  // planes q = 2 axis + part of this wave's 16 columns
