@@ -481,8 +481,10 @@ def bench_hbm_ceiling(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # defaults sized per workload below (None): enough warmup that the GPU
+    # reaches steady clocks before the timed region
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
@@ -495,6 +497,11 @@ def main():
                     help="grid of the per-frame kernel: 1 equal quad counts, 2 one wave "
                          "per resident slot (0 = library default, 2)")
     args = ap.parse_args()
+    per_workload = {"one-vs-all": (200, 100), "all-pairs": (5, 3), "batched": (50, 20),
+                    "hbm-ceiling": (20, 10), "serving": (0, 0)}
+    d_steps, d_warm = per_workload[args.workload]
+    args.steps = d_steps if args.steps is None else args.steps
+    args.warmup = d_warm if args.warmup is None else args.warmup
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "path is a design (frames shard across GPUs with no exchange in the hot loop), "
