@@ -2,8 +2,8 @@
 
 Pools are uploaded once and kept in HBM in prepared split form
 (``allpairs.SplitSet``: centred, hi/lo planes, traces).  A request carries 1 to
-64 query frames with the same atom count as its pool and asks for the RMSD of
-every query frame to every pool frame.
+``max_request_queries`` (64) query frames with the same atom count as its pool
+and asks for the RMSD of every query frame to every pool frame.
 
 A failure in the scheduler or completion thread (for example a failed
 allocation) fails every outstanding request: ``Request.wait`` raises it.
@@ -70,14 +70,26 @@ class Pool:
 
 class RMSDServer:
     """hbm_budget: bytes of HBM the server may hold (pools + batches in flight).
-    max_batch_queries: query frames per batch (<= 64 fills one MFMA row tile).
-    n_streams: batches that may be in flight at once."""
+    max_request_queries: query frames one request may carry.
+    max_batch_queries: query frames per batch; a batch reads its pool once
+    whatever its size (one MFMA row tile per 64 query frames), so under load
+    larger batches cost less per query.
+    n_streams: batches that may be in flight at once.
+    keep_results: copy every request's rows out of the reused pinned buffer
+    into its own array (False: only the arrival-to-rows-on-host latency is
+    recorded, for clients that consume the rows as they land)."""
 
-    def __init__(self, hbm_budget=200e9, mode="f16exact", max_batch_queries=64, n_streams=3):
+    def __init__(self, hbm_budget=200e9, mode="f16exact", max_batch_queries=256,
+                 max_request_queries=64, n_streams=3, keep_results=True):
         self.t = dev.require_gpu()
         self.mode = mode
         self.budget = int(hbm_budget)
         self.max_q = int(max_batch_queries)
+        self.max_req_q = int(max_request_queries)
+        if not 1 <= self.max_req_q <= self.max_q:
+            raise ValueError("need 1 <= max_request_queries <= max_batch_queries")
+        self.keep_results = bool(keep_results)
+        self.largest_batch = 0
         self.pools = []
         self.resident = 0
         self.inflight_bytes = 0
@@ -118,8 +130,8 @@ class RMSDServer:
         p = self.pools[pool_id]
         if q.shape[1] != p.n_atoms or q.shape[2] != 3:
             raise ValueError(f"queries must be (q, {p.n_atoms}, 3), got {q.shape}")
-        if not 1 <= q.shape[0] <= self.max_q:
-            raise ValueError(f"a request carries 1..{self.max_q} query frames")
+        if not 1 <= q.shape[0] <= self.max_req_q:
+            raise ValueError(f"a request carries 1..{self.max_req_q} query frames")
         r = Request(pool_id, q, time.perf_counter() if t_arrival is None else t_arrival)
         with self.cv:
             self.pending.append(r)
@@ -165,6 +177,7 @@ class RMSDServer:
                            and self.resident + self.inflight_bytes + need > self.budget):
                         self.cv.wait()
                     pool, batch, nq = self._take_batch()
+                    self.largest_batch = max(self.largest_batch, nq)
                     si = self.free_streams.popleft()
                     self.inflight_bytes += need
                 try:
@@ -221,7 +234,8 @@ class RMSDServer:
                 row = 0
                 for r in batch:
                     q = r.queries.shape[0]
-                    r.result = res[row:row + q].copy()   # the pinned buffer is reused
+                    if self.keep_results:
+                        r.result = res[row:row + q].copy()   # the pinned buffer is reused
                     row += q
                     r.t_done = now
                     r._done.set()

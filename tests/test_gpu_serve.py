@@ -32,20 +32,25 @@ def test_serve_matches_fp64_and_is_batch_independent(gpu):
             queries = pools[pid][sel] + rng.normal(0, 0.01, (q,) + pools[pid].shape[1:]).astype(np.float32)
             r = srv.submit(ids[pid], queries)
             probes.append((pid, queries, r.wait(60).copy()))
-        # in a mix: the same requests among many others, submitted together
-        mixed = []
-        for pid, queries, _ in probes:
-            for _ in range(5):
-                opid = int(rng.integers(0, len(pools)))
-                oq = int(rng.integers(1, 33))
-                srv.submit(ids[opid], pools[opid][rng.integers(0, pools[opid].shape[0], oq)])
-            mixed.append(srv.submit(ids[pid], queries))
+    # in a mix: the same requests among many others, all queued before a
+    # second server starts, so its batches merge requests (and row tiles)
+    srv = RMSDServer(hbm_budget=8e9)
+    ids = [srv.add_pool(p) for p in pools]
+    mixed = []
+    for pid, queries, _ in probes:
+        for _ in range(5):
+            opid = int(rng.integers(0, len(pools)))
+            oq = int(rng.integers(1, 33))
+            srv.submit(ids[opid], pools[opid][rng.integers(0, pools[opid].shape[0], oq)])
+        mixed.append(srv.submit(ids[pid], queries))
+    with srv:
         for (pid, queries, alone), r in zip(probes, mixed):
             got = r.wait(60)
             assert np.array_equal(got, alone)
             want = fp64.rmsd_matrix(np.concatenate([queries, pools[pid]]))[:len(queries), len(queries):]
             assert np.abs(got - want).max() < 1e-5
         assert srv.batches >= 2
+        assert srv.largest_batch > 64   # requests merged across MFMA row tiles
 
 
 def test_serve_budget(gpu):

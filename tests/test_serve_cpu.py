@@ -1,0 +1,133 @@
+"""The serving loop's host logic on the CPU: continuous batching across
+requests and MFMA row tiles, results routed to the right requests, failure
+propagation and the HBM budget.  The GPU pieces (streams, events, the split
+prep and the cross-Gram) are replaced by CPU stand-ins whose results come
+from the FP64 oracle, so only the scheduling code is under test here; the
+GPU path is tested in test_gpu_serve.py."""
+import contextlib
+
+import numpy as np
+import pytest
+import torch
+
+from mdtraj_amd import gen, serve
+from oracle import fp64
+
+
+class _Cuda:
+    class Stream:
+        pass
+
+    @staticmethod
+    @contextlib.contextmanager
+    def stream(s):
+        yield
+
+    class Event:
+        def record(self, stream=None):
+            pass
+
+        def synchronize(self):
+            pass
+
+
+class _Torch:
+    cuda = _Cuda
+    float32 = torch.float32
+
+    def __getattr__(self, name):
+        return getattr(torch, name)
+
+    @staticmethod
+    def empty(*a, pin_memory=False, **k):
+        return torch.empty(*a, **k)
+
+
+class _Split:
+    def __init__(self, xyz, mode="f16exact"):
+        self.x = np.asarray(xyz, np.float32)
+        self.n_frames = self.x.shape[0]
+        self.nbytes = self.x.nbytes
+
+
+def _cross(qs, ps):
+    rows = [fp64.rmsd(ps.x, qs.x, i) for i in range(qs.n_frames)]
+    return torch.from_numpy(np.asarray(rows, np.float32))
+
+
+@pytest.fixture
+def cpu_server(monkeypatch):
+    monkeypatch.setattr(serve.dev, "require_gpu", lambda: _Torch())
+    monkeypatch.setattr(serve.dev, "device", lambda: "cpu")
+    monkeypatch.setattr(serve.dev, "to_device_f32",
+                        lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)))
+    monkeypatch.setattr(serve, "SplitSet", _Split)
+    monkeypatch.setattr(serve, "rmsd_cross", _cross)
+    return serve.RMSDServer
+
+
+def test_batches_merge_and_rows_reach_their_requests(cpu_server):
+    pools = [gen.walk(seed=20 + i, n_frames=f, n_atoms=n)
+             for i, (f, n) in enumerate([(40, 12), (25, 30)])]
+    rng = np.random.default_rng(1)
+    srv = cpu_server(hbm_budget=1e9, max_batch_queries=128)
+    ids = [srv.add_pool(p) for p in pools]
+    reqs = []
+    for _ in range(30):
+        pid = int(rng.integers(0, 2))
+        q = int(rng.integers(1, 65))
+        queries = pools[pid][rng.integers(0, pools[pid].shape[0], q)]
+        reqs.append((pid, queries, srv.submit(ids[pid], queries)))
+    with srv:
+        for pid, queries, r in reqs:
+            got = r.wait(30)
+            want = np.asarray([fp64.rmsd(pools[pid], queries, i) for i in range(len(queries))])
+            assert got.shape == (len(queries), pools[pid].shape[0])
+            assert np.abs(got - want).max() < 1e-6
+            assert r.latency is not None and r.latency >= 0
+    assert 64 < srv.largest_batch <= 128
+    assert srv.batches < len(reqs)
+
+
+def test_request_limits(cpu_server):
+    srv = cpu_server(hbm_budget=1e9)
+    pid = srv.add_pool(gen.walk(seed=1, n_frames=10, n_atoms=8))
+    with pytest.raises(ValueError, match="1..64"):
+        srv.submit(pid, np.zeros((65, 8, 3), np.float32))
+    with pytest.raises(ValueError):
+        srv.submit(pid, np.zeros((2, 9, 3), np.float32))
+    with pytest.raises(ValueError):
+        cpu_server(max_batch_queries=32, max_request_queries=64)
+
+
+def test_budget_refuses_pools(cpu_server):
+    srv = cpu_server(hbm_budget=10_000)
+    with pytest.raises(MemoryError):
+        srv.add_pool(gen.walk(seed=1, n_frames=500, n_atoms=30))
+
+
+def test_without_copy_out_only_latency_is_recorded(cpu_server):
+    srv = cpu_server(hbm_budget=1e9, keep_results=False)
+    pool = gen.walk(seed=2, n_frames=20, n_atoms=10)
+    pid = srv.add_pool(pool)
+    with srv:
+        r = srv.submit(pid, pool[:3])
+        assert r.wait(30) is None
+        assert r.latency is not None
+
+
+def test_failure_reaches_every_waiter(cpu_server, monkeypatch):
+    def boom(*a, **k):
+        raise MemoryError("forced failure")
+
+    srv = cpu_server(hbm_budget=1e9)
+    pool = gen.walk(seed=3, n_frames=30, n_atoms=10)
+    pid = srv.add_pool(pool)
+    monkeypatch.setattr(serve, "rmsd_cross", boom)
+    reqs = [srv.submit(pid, pool[:2]) for _ in range(6)]
+    with srv:
+        for r in reqs:
+            with pytest.raises(RuntimeError):
+                r.wait(30)
+        with pytest.raises(RuntimeError):
+            srv.submit(pid, pool[:1])
