@@ -233,6 +233,52 @@ __device__ __forceinline__ void row_reduce(Sums& s) {
     }
 }
 
+// Finish a group of cnt frames whose row sums are parked in s_res: one lane
+// per frame computes the QCP MSD (or the fit, or the no-superpose MSD).
+template <Kind kKind>
+__device__ __forceinline__ void finish_group(long long base, int cnt, double n,
+                                             const double (&rs)[4], long long self_frame,
+                                             float* __restrict__ out, float* __restrict__ rot_out,
+                                             double* __restrict__ mu_out,
+                                             double* __restrict__ s_res, int lane) {
+    constexpr int kNS = SumsPerFrame<kKind>::value;
+    // the row leaders' LDS stores must be visible to every lane of the wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < cnt) {
+        const long long f = base + lane;
+        Sums mine;
+        load_sums<kKind>(mine, s_res + lane * kNS);
+        if constexpr (kKind == Kind::NoSuperpose) {
+            out[f] = (float)sqrt(mine.q / n);
+        } else {
+            double M[9], ga, gb;
+            centred(mine, rs, n, M, ga, gb);
+            if constexpr (kKind == Kind::Rmsd) {
+                const double msd = qcp_msd(M, ga, gb, n);
+                out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
+            } else {
+                double R[9];
+                const double lam = qcp_solve<true>(M, 0.5 * (ga + gb), R);
+#pragma unroll
+                for (int e = 0; e < 9; ++e) rot_out[9 * f + e] = (float)R[e];
+                mu_out[3 * f + 0] = mine.sx / n;
+                mu_out[3 * f + 1] = mine.sy / n;
+                mu_out[3 * f + 2] = mine.sz / n;
+                if (out) {
+                    const double msd = (ga + gb - 2.0 * lam) / n;
+                    out[f] = (float)sqrt(msd > 0.0 ? msd : 0.0);
+                }
+            }
+        }
+    }
+    // the next group overwrites s_res: all lanes must have read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One group of up to 64 consecutive frames (base .. base + cnt) of one
 // trajectory through the wave: items (frame quad, atom block) stream through
 // the register ring, each quad's row sums are reduced (DPP) and parked in the
@@ -299,42 +345,106 @@ __device__ __forceinline__ void process_group(
             }
         }
     }
-    // the row leaders' LDS stores must be visible to every lane of the wave
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < cnt) {
-        const long long f = base + lane;
-        Sums mine;
-        load_sums<kKind>(mine, s_res + lane * kNS);
-        if constexpr (kKind == Kind::NoSuperpose) {
-            out[f] = (float)sqrt(mine.q / n);
-        } else {
-            double M[9], ga, gb;
-            centred(mine, rs, n, M, ga, gb);
-            if constexpr (kKind == Kind::Rmsd) {
-                const double msd = qcp_msd(M, ga, gb, n);
-                out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
-            } else {
-                double R[9];
-                const double lam = qcp_solve<true>(M, 0.5 * (ga + gb), R);
+    finish_group<kKind>(base, cnt, n, rs, self_frame, out, rot_out, mu_out, s_res, lane);
+}
+
+// Batched variant of the item loop (one system's frames, all atoms, no
+// gather).  Each wave works on its own system, so the reference is not
+// staged in LDS up front: every item's block of 64 reference atoms rides in
+// the load ring with the frame data (lane l loads atom 64 b + l of the three
+// planes, 2 items ahead) and is handed to the lanes that use it through a
+// 768-byte per-wave LDS buffer.  Same arithmetic as process_group.
+constexpr int kRingB = 3;
+
+__device__ __forceinline__ void load_item_batch(const float* __restrict__ fr, bool row_ok,
+                                                int n, int t, int b, int lane,
+                                                const float* __restrict__ rx,
+                                                const float* __restrict__ ry,
+                                                const float* __restrict__ rz,
+                                                float (&ax)[kItemAtoms], float (&ay)[kItemAtoms],
+                                                float (&az)[kItemAtoms], float& qx, float& qy,
+                                                float& qz) {
+    load_item<false, false>(fr, row_ok, 0, nullptr, n, t, b, ax, ay, az);
+    const int jr = b * kRowLanes * kItemAtoms + lane;
+    qx = 0.0f;
+    qy = 0.0f;
+    qz = 0.0f;
+    if (jr < n) {
+        qx = rx[jr];
+        qy = ry[jr];
+        qz = rz[jr];
+    }
+}
+
+__device__ __forceinline__ void process_group_batch(
+    const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
+    const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
+    const double (&rs)[4], long long self_frame, float* __restrict__ out,
+    double* __restrict__ s_res, float* __restrict__ s_ref, int lane) {
+    constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
+    constexpr int kBlk = kRowLanes * kItemAtoms;  // atoms per item block
+    const int row = lane / kRowLanes, t = lane % kRowLanes;
+    const long long frame_stride = 3LL * n_atoms;
+    const int n_blk = (n_atoms + kBlk - 1) / kBlk;
+    const int n_quads = (cnt + 3) / 4;
+    const int n_items = n_quads * n_blk;
+    float bx[kRingB][kItemAtoms], by[kRingB][kItemAtoms], bz[kRingB][kItemAtoms];
+    float qx[kRingB], qy[kRingB], qz[kRingB];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) rot_out[9 * f + e] = (float)R[e];
-                mu_out[3 * f + 0] = mine.sx / n;
-                mu_out[3 * f + 1] = mine.sy / n;
-                mu_out[3 * f + 2] = mine.sz / n;
-                if (out) {
-                    const double msd = (ga + gb - 2.0 * lam) / n;
-                    out[f] = (float)sqrt(msd > 0.0 ? msd : 0.0);
+    for (int r = 0; r < kRingB - 1; ++r) {
+        if (r < n_items) {
+            const int q = r / n_blk, b = r - q * n_blk;
+            const int k = 4 * q + row;
+            load_item_batch(xyz + (base + k) * frame_stride, k < cnt, n_atoms, t, b, lane, rx,
+                            ry, rz, bx[r], by[r], bz[r], qx[r], qy[r], qz[r]);
+        }
+    }
+    Sums s = {};
+    for (int it0 = 0; it0 < n_items; it0 += kRingB) {
+#pragma unroll
+        for (int r = 0; r < kRingB; ++r) {
+            const int it = it0 + r;
+            if (it < n_items) {
+                const int nxt = it + kRingB - 1;
+                if (nxt < n_items) {
+                    const int q = nxt / n_blk, b = nxt - q * n_blk;
+                    const int k = 4 * q + row;
+                    const int slot = (r + kRingB - 1) % kRingB;
+                    load_item_batch(xyz + (base + k) * frame_stride, k < cnt, n_atoms, t, b, lane,
+                                    rx, ry, rz, bx[slot], by[slot], bz[slot], qx[slot], qy[slot],
+                                    qz[slot]);
+                }
+                // hand the item's reference block to its users (same wave:
+                // LDS operations complete in program order)
+                s_ref[lane] = qx[r];
+                s_ref[kBlk + lane] = qy[r];
+                s_ref[2 * kBlk + lane] = qz[r];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int q = it / n_blk, b = it - q * n_blk;
+#pragma unroll
+                for (int u = 0; u < kItemAtoms; ++u) {
+                    const int o = t + u * kRowLanes;
+                    accumulate<Kind::Rmsd>(s, bx[r][u], by[r][u], bz[r][u], s_ref[o],
+                                           s_ref[kBlk + o], s_ref[2 * kBlk + o]);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (b == n_blk - 1) {
+                    row_reduce<Kind::Rmsd>(s);
+                    const int k = 4 * q + row;
+                    if (t == 0 && k < cnt) store_sums<Kind::Rmsd>(s_res + k * kNS, s);
+                    s = Sums{};
                 }
             }
         }
     }
-    // the next group overwrites s_res: all lanes must have read it
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    finish_group<Kind::Rmsd>(base, cnt, (double)n_atoms, rs, self_frame, out, nullptr, nullptr,
+                             s_res, lane);
 }
+
 
 }  // namespace
 }  // namespace hr
