@@ -87,6 +87,19 @@ __device__ __forceinline__ double seg_sum(double v, int width) {
     return v;
 }
 
+// 1/d.  On the GPU: the hardware estimate (v_rcp_f64) refined by one
+// Newton-Raphson step, r + r (1 - d r), in place of the IEEE division
+// sequence; the Newton iteration below corrects any last-ulp difference.
+// The host build of this code (used by the CPU tests) divides.
+__host__ __device__ __forceinline__ double recip(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(d);
+    return fma(r, fma(-d, r, 1.0), r);
+#else
+    return 1.0 / d;
+#endif
+}
+
 // Largest real root of P(x) = x^4 + c2 x^2 + c1 x + c0 whose roots are all
 // real (P is the characteristic polynomial of the symmetric key matrix K).
 // Newton from a start that lies above every root converges monotonically to
@@ -109,59 +122,81 @@ __device__ __forceinline__ double seg_sum(double v, int width) {
 // (near-)double roots -- the two largest eigenvalues of K (nearly) coincide,
 // as for a centred two-atom (collinear) structure -- where P's values
 // determine the root only to about sqrt(eps) lambda.
-// 1/d.  On the GPU: the hardware estimate (v_rcp_f64) refined by one
-// Newton-Raphson step, r + r (1 - d r), in place of the IEEE division
-// sequence; the Newton iteration below corrects any last-ulp difference.
-// The host build of this code (used by the CPU tests) divides.
-__host__ __device__ __forceinline__ double recip(double d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double r = __builtin_amdgcn_rcp(d);
-    return fma(r, fma(-d, r, 1.0), r);
-#else
-    return 1.0 / d;
-#endif
+// Coefficients of P(x) = x^4 + c2 x^2 + c1 x + c0 and the Frobenius bound.
+struct Quartic {
+    double c2, c1, c0, frob;
+};
+
+// Newton state: the last point verified above every root, P, P', P'' there;
+// run: the iteration continues.
+struct NewtonState {
+    double x, p, d1, d2;
+    bool run;
+};
+
+__host__ __device__ __forceinline__ void quartic_eval(const Quartic& q, double t, double& p,
+                                                     double& d1, double& d2) {
+    const double t2 = t * t;
+    p = (t2 + q.c2) * t2 + q.c1 * t + q.c0;
+    d1 = (4.0 * t2 + 2.0 * q.c2) * t + q.c1;
+    d2 = 12.0 * t2 + 2.0 * q.c2;
+}
+
+// Start at e0, else at the Frobenius bound; run is false when neither is
+// verified above the roots (then the root is left to the Jacobi fallback).
+__host__ __device__ __forceinline__ NewtonState newton_start(const Quartic& q, double e0) {
+    NewtonState s;
+    s.x = e0;
+    quartic_eval(q, s.x, s.p, s.d1, s.d2);
+    s.run = s.p > 0.0 && s.d1 > 0.0 && s.d2 > 0.0 && s.x > 0.0;
+    if (!s.run) {
+        s.x = 2.0 * q.frob * (1.0 + 1e-12) + 1e-300;
+        quartic_eval(q, s.x, s.p, s.d1, s.d2);
+        s.run = s.p > 0.0 && s.d1 > 0.0 && s.d2 > 0.0 && s.x > 0.0;
+    }
+    return s;
+}
+
+// One Newton step, branch-free (so that two solves can be interleaved): the
+// new point is taken only if it is still verified above the roots; the
+// iteration stops once a step falls below 1e-8 x -- quadratic convergence
+// then leaves the final estimate x - P/P' about 1e-16 x from the root, which
+// the bracket of newton_finish certifies (or not, at a near-double root).
+__host__ __device__ __forceinline__ void newton_step(const Quartic& q, NewtonState& s) {
+    const double nx = s.x - s.p * recip(s.d1);
+    double np, nd1, nd2;
+    quartic_eval(q, nx, np, nd1, nd2);
+    const bool take = s.run && nx < s.x && np > 0.0 && nd1 > 0.0 && nd2 > 0.0 && nx > 0.0;
+    const double step = s.x - nx;
+    s.x = take ? nx : s.x;
+    s.p = take ? np : s.p;
+    s.d1 = take ? nd1 : s.d1;
+    s.d2 = take ? nd2 : s.d2;
+    s.run = take && step > 1e-8 * nx;
+}
+
+// Final estimate y = x - P/P' and whether its bracket certifies it:
+// width 8 P'' q^2 / P'^3 + 2 perr / P' (q = P + perr, perr the Horner
+// rounding bound of P) within 1e-11 x, multiplied through by P'^3 (an
+// underflowing P'^3 counts as unresolved).
+__host__ __device__ __forceinline__ double newton_finish(const Quartic& q, const NewtonState& s,
+                                                        bool started, bool* resolved) {
+    const double x = s.x, x2 = x * x;
+    const double perr = 8.9e-16 * (x2 * x2 + fabs(q.c2) * x2 + fabs(q.c1) * x + fabs(q.c0));
+    const double qq = s.p + perr;
+    const double d13 = s.d1 * s.d1 * s.d1;
+    *resolved = started && d13 > 0.0 &&
+                8.0 * s.d2 * qq * qq + 2.0 * perr * s.d1 * s.d1 <= 1e-11 * x * d13;
+    return x - s.p * recip(s.d1);
 }
 
 __host__ __device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
                                                    double m_frob, bool* resolved) {
-    double x = e0, p, d1, d2;
-    auto eval = [&](double t, double& pp, double& q1, double& q2) {
-        const double t2 = t * t;
-        pp = (t2 + c2) * t2 + c1 * t + c0;
-        q1 = (4.0 * t2 + 2.0 * c2) * t + c1;
-        q2 = 12.0 * t2 + 2.0 * c2;
-    };
-    eval(x, p, d1, d2);
-    if (!(p > 0.0 && d1 > 0.0 && d2 > 0.0 && x > 0.0)) {
-        x = 2.0 * m_frob * (1.0 + 1e-12) + 1e-300;
-        eval(x, p, d1, d2);
-        if (!(p > 0.0 && d1 > 0.0 && d2 > 0.0 && x > 0.0)) {
-            *resolved = false;
-            return x;
-        }
-    }
-    for (int it = 0; it < 64; ++it) {
-        const double nx = x - p * recip(d1);
-        if (!(nx < x)) break;
-        double np, nd1, nd2;
-        eval(nx, np, nd1, nd2);
-        if (!(np > 0.0 && nd1 > 0.0 && nd2 > 0.0 && nx > 0.0)) break;
-        const double step = x - nx;
-        x = nx;
-        p = np;
-        d1 = nd1;
-        d2 = nd2;
-        if (step <= 1e-15 * x) break;
-    }
-    const double x2 = x * x;
-    // Horner rounding bound of P at x: 4 eps times the sum of |terms|
-    const double perr = 8.9e-16 * (x2 * x2 + fabs(c2) * x2 + fabs(c1) * x + fabs(c0));
-    // bracket width 8 P'' q^2 / P'^3 + 2 perr / P' (q = P + perr) against 1e-11 x,
-    // multiplied through by P'^3; an underflowing P'^3 counts as unresolved
-    const double q = p + perr;
-    const double d13 = d1 * d1 * d1;
-    *resolved = d13 > 0.0 && 8.0 * d2 * q * q + 2.0 * perr * d1 * d1 <= 1e-11 * x * d13;
-    return x - p * recip(d1);
+    const Quartic q{c2, c1, c0, m_frob};
+    NewtonState s = newton_start(q, e0);
+    const bool started = s.run;
+    for (int it = 0; it < 64 && s.run; ++it) newton_step(q, s);
+    return newton_finish(q, s, started, resolved);
 }
 
 // Key-matrix entries of Horn/Theobald for M[r*3+c] = sum a_r b_c, in the
@@ -313,18 +348,15 @@ __host__ __device__ inline bool qcp_rotation(const KeyMatrix& k, double lam, dou
 // C2 = -2 |M|_F^2, C1 = -8 det M, C0 = det K.  Newton on P is the fast path;
 // an unresolved (near-double) root, or a rank-deficient K - lambda I for the
 // rotation, is settled by Jacobi on K itself.
-template <bool kRot>
-__host__ __device__ inline double qcp_solve(const double M[9], double e0, double R[9]) {
-    const KeyMatrix k = key_matrix(M);
+// Characteristic polynomial of K(M): C2 = -2 |M|_F^2, C1 = -8 det M,
+// C0 = det K (cofactor expansion along the first row).
+__host__ __device__ __forceinline__ Quartic qcp_quartic(const double M[9], const KeyMatrix& k) {
     double mm = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) mm = fma(M[i], M[i], mm);
-    const double c2 = -2.0 * mm;
     const double det_m = M[0] * (M[4] * M[8] - M[5] * M[7]) -
                          M[1] * (M[3] * M[8] - M[5] * M[6]) +
                          M[2] * (M[3] * M[7] - M[4] * M[6]);
-    const double c1 = -8.0 * det_m;
-    // det K by cofactor expansion along the first row
     const double a = k.k22 * k.k33 - k.k23 * k.k23;
     const double b = k.k12 * k.k33 - k.k13 * k.k23;
     const double c = k.k12 * k.k23 - k.k13 * k.k22;
@@ -335,19 +367,26 @@ __host__ __device__ inline double qcp_solve(const double M[9], double e0, double
                       k.k01 * (k.k01 * a - k.k12 * d + k.k13 * e) +
                       k.k02 * (k.k01 * b - k.k11 * d + k.k13 * f) -
                       k.k03 * (k.k01 * c - k.k11 * e + k.k12 * f);
+    return Quartic{-2.0 * mm, -8.0 * det_m, c0, sqrt(mm)};
+}
+
+template <bool kRot>
+__host__ __device__ inline double qcp_solve(const double M[9], double e0, double R[9]) {
+    const KeyMatrix k = key_matrix(M);
+    const Quartic q = qcp_quartic(M, k);
     bool resolved = true;
-    double lam = qcp_largest_root(c2, c1, c0, e0, sqrt(mm), &resolved);
+    double lam = qcp_largest_root(q.c2, q.c1, q.c0, e0, q.frob, &resolved);
     if constexpr (kRot) {
         if (resolved && qcp_rotation(k, lam, R)) return lam;
-        if (!(mm > 0.0)) {
+        if (!(q.frob > 0.0)) {
             quat_rotation(1.0, 0.0, 0.0, 0.0, R);
             return 0.0;
         }
-        double q[4];
-        lam = qcp_jacobi<true>(k, q);
-        const double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        double qv[4];
+        lam = qcp_jacobi<true>(k, qv);
+        const double qq = qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3];
         const double inv = 1.0 / sqrt(qq);
-        quat_rotation(q[0] * inv, q[1] * inv, q[2] * inv, q[3] * inv, R);
+        quat_rotation(qv[0] * inv, qv[1] * inv, qv[2] * inv, qv[3] * inv, R);
         return lam;
     } else {
         if (!resolved) lam = qcp_jacobi<false>(k, nullptr);
@@ -361,6 +400,32 @@ __host__ __device__ __forceinline__ double qcp_msd(const double M[9], double ga,
     const double lam = qcp_solve<false>(M, 0.5 * (ga + gb), nullptr);
     const double msd = (ga + gb - 2.0 * lam) / n;
     return msd > 0.0 ? msd : 0.0;
+}
+
+// Two independent MSDs with their Newton iterations interleaved step by step
+// (the branch-free steps of the two solves overlap in the pipeline, where a
+// single solve is a chain of dependent operations); same results as two
+// qcp_msd calls.  An unresolved root takes the Jacobi fallback on its own.
+__host__ __device__ inline void qcp_msd2(const double Ma[9], double gaa, double gba,
+                                         const double Mb[9], double gab, double gbb, double n,
+                                         double& msd_a, double& msd_b) {
+    const KeyMatrix ka = key_matrix(Ma), kb = key_matrix(Mb);
+    const Quartic qa = qcp_quartic(Ma, ka), qb = qcp_quartic(Mb, kb);
+    NewtonState sa = newton_start(qa, 0.5 * (gaa + gba));
+    NewtonState sb = newton_start(qb, 0.5 * (gab + gbb));
+    const bool started_a = sa.run, started_b = sb.run;
+    for (int it = 0; it < 64 && (sa.run || sb.run); ++it) {
+        newton_step(qa, sa);
+        newton_step(qb, sb);
+    }
+    bool res_a, res_b;
+    double la = newton_finish(qa, sa, started_a, &res_a);
+    double lb = newton_finish(qb, sb, started_b, &res_b);
+    if (!res_a) la = qcp_jacobi<false>(ka, nullptr);
+    if (!res_b) lb = qcp_jacobi<false>(kb, nullptr);
+    const double ma = (gaa + gba - 2.0 * la) / n, mb = (gab + gbb - 2.0 * lb) / n;
+    msd_a = ma > 0.0 ? ma : 0.0;
+    msd_b = mb > 0.0 ? mb : 0.0;
 }
 
 }  // namespace hr

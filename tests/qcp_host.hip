@@ -8,6 +8,11 @@ double qcp_msd_host(const double* M, double ga, double gb, double n) {
     return hr::qcp_msd(M, ga, gb, n);
 }
 
+void qcp_msd2_host(const double* Ma, double gaa, double gba, const double* Mb, double gab,
+                   double gbb, double n, double* out) {
+    hr::qcp_msd2(Ma, gaa, gba, Mb, gab, gbb, n, out[0], out[1]);
+}
+
 double qcp_rot_host(const double* M, double ga, double gb, double* R) {
     return hr::qcp_solve<true>(M, 0.5 * (ga + gb), R);
 }

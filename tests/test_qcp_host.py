@@ -39,6 +39,9 @@ def qcp():
     lib.qcp_msd_host.argtypes = [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     lib.qcp_rot_host.restype = ctypes.c_double
     lib.qcp_rot_host.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P]
+    lib.qcp_msd2_host.restype = None
+    lib.qcp_msd2_host.argtypes = [_P, ctypes.c_double, ctypes.c_double, _P, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, _P]
     lib.qcp_newton_resolved_host.restype = ctypes.c_int
     lib.qcp_newton_resolved_host.argtypes = [_P, ctypes.c_double, ctypes.c_double]
     return lib
@@ -85,6 +88,19 @@ def test_msd_matches_eigensolver_on_degenerate_spectra(qcp):
     # FP64 raw sums of float32 coordinates up to ~25 nm: r near 0 carries
     # sqrt(eps * G / N) noise, far below the 1e-5 nm bar
     assert all(v < 2e-6 for v in worst.values()), worst
+
+
+def test_interleaved_pair_solve_equals_two_single_solves(qcp):
+    """qcp_msd2 (the all-pairs epilogue's two interleaved solves) returns
+    exactly what two qcp_msd calls return, Jacobi fallbacks included."""
+    cases = [(_sums(a, b)) for _, a, b in _structures(3, 400)]
+    for (Ma, gaa, gba, n), (Mb, gab, gbb, _) in zip(cases[0::2], cases[1::2]):
+        out = np.zeros(2)
+        qcp.qcp_msd2_host(Ma.ctypes.data_as(_P), gaa, gba, Mb.ctypes.data_as(_P), gab, gbb,
+                          float(n), out.ctypes.data_as(_P))
+        a = qcp.qcp_msd_host(Ma.ctypes.data_as(_P), gaa, gba, float(n))
+        b = qcp.qcp_msd_host(Mb.ctypes.data_as(_P), gab, gbb, float(n))
+        assert out[0] == a and out[1] == b
 
 
 def test_rotation_is_proper_and_optimal(qcp):
