@@ -189,6 +189,18 @@ def bench_one_vs_all(args):
             "traffic": traffic_from_profile("one_vs_all_traffic.json"),
         },
     }
+    # outside the timed region: the same query through md.rmsd with the
+    # trajectory in host memory (one 2.4 GB upload per call)
+    import mdtraj_amd as md
+
+    host = md.Trajectory(X.cpu().numpy())
+    md.rmsd(host, host, 0, atom_indices=idx)
+    t0 = time.perf_counter()
+    md.rmsd(host, host, 0, atom_indices=idx)
+    host_s = time.perf_counter() - t0
+    result["config"]["host_input_frame_pairs_per_s"] = F / host_s
+    log(f"md.rmsd with host arrays (upload included): {host_s * 1e3:.1f} ms")
+    del host
     if not args.no_cpu_baseline:
         log("cpu baseline (reference core on host cores)")
         cb, ref_out = cpu_baseline_one_vs_all(X, idx)

@@ -11,8 +11,8 @@
 //      wave takes the next unit from the ticket (vector atomic, the next one
 //      requested while the current unit runs) until the table is exhausted,
 //      and runs the per-frame code on it (16 lanes per frame, FP64 sums, the
-//      reference streamed through the load ring with the frames, QCP one lane
-//      per frame).
+//      reference streamed through the load ring with the frames and passed
+//      between lanes by permutes, QCP one lane per frame).
 // Every wave leaves when the ticket passes the table's end, so the grid
 // drains on every launch.  A plan must not be launched on two streams at once
 // (the ticket is shared).
@@ -69,9 +69,6 @@ __global__ __launch_bounds__(kBlock) void batch_ref_prep_kernel(const DevSystem*
                   s.ref_sel, s.ref_stats);
 }
 
-// per-wave LDS hand-off buffer of one reference atom block (3 planes x 64)
-constexpr int kRefBlockFloats = 3 * kRowLanes * kItemAtoms;
-
 template <int kMinWaves>
 __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     const DevSystem* __restrict__ sys, const DevUnit* __restrict__ units, long long n_units,
@@ -81,8 +78,6 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     double* s_res = smem_d + wv * kWave * kNS;
-    float* s_ref = reinterpret_cast<float*>(smem_d + kWavesPerBlock * kWave * kNS) +
-                   wv * kRefBlockFloats;
     unsigned u = 0;
     if (lane == 0) u = atomicAdd(ticket, 1u);
     u = __shfl(u, 0, kWave);
@@ -95,8 +90,7 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
         const int n = s.n_atoms;
         const double rs[4] = {s.ref_stats[0], s.ref_stats[1], s.ref_stats[2], s.ref_stats[3]};
         process_group_batch(s.xyz, unit.base, unit.cnt, n, s.ref_sel, s.ref_sel + n,
-                            s.ref_sel + 2 * n, rs, (long long)s.ref_frame, s.out, s_res, s_ref,
-                            lane);
+                            s.ref_sel + 2 * n, rs, (long long)s.ref_frame, s.out, s_res, lane);
         u = __shfl(next, 0, kWave);
     }
 }
@@ -119,8 +113,7 @@ template <int kMinWaves>
 int capture_plan(hr_batch_plan* p) {
     auto k = batch_kernel<kMinWaves>;
     const size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
-                           sizeof(double) +
-                       (size_t)kWavesPerBlock * kRefBlockFloats * sizeof(float);
+                       sizeof(double);
     int per_cu = 0;
     HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(k), kBlock, lds));

@@ -352,8 +352,8 @@ __device__ __forceinline__ void process_group(
 // gather).  Each wave works on its own system, so the reference is not
 // staged in LDS up front: every item's block of 64 reference atoms rides in
 // the load ring with the frame data (lane l loads atom 64 b + l of the three
-// planes, 2 items ahead) and is handed to the lanes that use it through a
-// 768-byte per-wave LDS buffer.  Same arithmetic as process_group.
+// planes, 2 items ahead) and each lane fetches the four atoms it uses from
+// the lanes that loaded them.  Same arithmetic as process_group.
 constexpr int kRingB = 3;
 
 __device__ __forceinline__ void load_item_batch(const float* __restrict__ fr, bool row_ok,
@@ -380,7 +380,7 @@ __device__ __forceinline__ void process_group_batch(
     const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
     const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
     const double (&rs)[4], long long self_frame, float* __restrict__ out,
-    double* __restrict__ s_res, float* __restrict__ s_ref, int lane) {
+    double* __restrict__ s_res, int lane) {
     constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
     constexpr int kBlk = kRowLanes * kItemAtoms;  // atoms per item block
     const int row = lane / kRowLanes, t = lane % kRowLanes;
@@ -414,24 +414,16 @@ __device__ __forceinline__ void process_group_batch(
                                     rx, ry, rz, bx[slot], by[slot], bz[slot], qx[slot], qy[slot],
                                     qz[slot]);
                 }
-                // hand the item's reference block to its users (same wave:
-                // LDS operations complete in program order)
-                s_ref[lane] = qx[r];
-                s_ref[kBlk + lane] = qy[r];
-                s_ref[2 * kBlk + lane] = qz[r];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // atom 64 b + t + 16 u of the reference was loaded by lane
+                // 16 u + t: fetch it from there (cross-lane permute)
                 const int q = it / n_blk, b = it - q * n_blk;
 #pragma unroll
                 for (int u = 0; u < kItemAtoms; ++u) {
-                    const int o = t + u * kRowLanes;
-                    accumulate<Kind::Rmsd>(s, bx[r][u], by[r][u], bz[r][u], s_ref[o],
-                                           s_ref[kBlk + o], s_ref[2 * kBlk + o]);
+                    const int src = u * kRowLanes + t;
+                    accumulate<Kind::Rmsd>(s, bx[r][u], by[r][u], bz[r][u],
+                                           __shfl(qx[r], src, kWave), __shfl(qy[r], src, kWave),
+                                           __shfl(qz[r], src, kWave));
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 if (b == n_blk - 1) {
                     row_reduce<Kind::Rmsd>(s);
                     const int k = 4 * q + row;
