@@ -32,3 +32,49 @@ def test_host_only_entry_points():
     assert lib.hr_version() >= 100
     assert lib.hr_rmsd_workspace_bytes(1500) >= 1500 * 12
     assert lib.hr_superpose_workspace_bytes(1000, 100) >= 1000 * (24 + 36)
+
+
+def _status(lib, name, *args):
+    rc = getattr(lib, name)(*args)
+    return rc, (lib.hr_last_error() or b"").decode()
+
+
+def test_invalid_arguments_are_rejected_on_the_host():
+    """Argument checks run before any device work, so they answer here:
+    a positive HR_ERR_* status and a message naming the problem."""
+    lib = _lib.load()
+    HR_ERR_INVALID = 1
+    ws = ctypes.create_string_buffer(64)
+    cases = [
+        ("hr_rmsd_one_vs_all", (None, -1, 10, None, 10, None, None, -1, None, None, None),
+         "invalid sizes"),
+        ("hr_rmsd_one_vs_all", (None, 5, 10, None, 7, None, None, -1, None, None, None),
+         "n_sel"),
+        ("hr_rmsd_one_vs_all", (None, 5, 10, None, 10, None, None, -1, None, None, None),
+         "NULL"),
+        ("hr_rmsd_all_pairs", (None, 10, 5, None, 5, 9, None, None, 0, None), "unknown mode"),
+        ("hr_rmsd_cross", (None, 2, None, 2, 5, 0, None, 2, None), "not a split mode"),
+        ("hr_rmsf", (None, 0, 5, None, 5, None, None, 1, None, None, None), "invalid sizes"),
+        ("hr_rmsd_align_displace", (None, 3, 2, 4, None, None, 4, 4, None, None, None, None,
+                                    None), "invalid sizes"),
+        ("hr_batch_plan_create", (None, 0, None, ctypes.addressof(ws)), "no systems"),
+        ("hr_set_tuning", (7, 1), "unknown tuning knob"),
+        ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, 5), "must be 0, 3 or 4"),
+        ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_GRID, 3), "must be 0, 1 or 2"),
+    ]
+    for name, args, text in cases:
+        rc, msg = _status(lib, name, *args)
+        assert rc == HR_ERR_INVALID, (name, rc, msg)
+        assert text in msg, (name, msg)
+
+
+def test_batch_shape_is_validated():
+    lib = _lib.load()
+    desc = (_lib.SystemDesc * 1)(_lib.SystemDesc(None, 10, 5, 0, None))
+    plan = ctypes.c_void_p()
+    for shape, text in (((65, 0, 3), "frames_per_unit"), ((8, -1, 3), "unit_kib"),
+                        ((8, 0, 5), "waves_per_simd")):
+        s = _lib.BatchShape(*shape)
+        rc, msg = _status(lib, "hr_batch_plan_create", ctypes.addressof(desc), 1,
+                          ctypes.addressof(s), ctypes.byref(plan))
+        assert rc == 1 and text in msg, (shape, rc, msg)

@@ -275,3 +275,24 @@ def test_device_resident_returns_tensor(gpu, golden):
     got = md.rmsd(t, t, 1)
     assert got.is_cuda and got.dtype == gpu.float32
     assert np.abs(got.cpu().numpy() - fp64.rmsd(w, w, 1)).max() < TOL_FP64
+
+
+def test_prepared_plan_vs_fp64(gpu):
+    """mdtraj_amd.plan.OneVsAll (hr_rmsd_prepare_reference once, then
+    hr_rmsd_one_vs_all_prepared per run): several references through one
+    plan, an external reference with its own indices, self frame exactly 0."""
+    from mdtraj_amd.plan import OneVsAll
+
+    x = gen.walk(seed=21, n_frames=500, n_atoms=300)
+    sel = gen.subset(5, 300, 220)
+    plan = OneVsAll(x, sel)
+    for f in (0, 123, 499):
+        got = plan.set_reference_frame(f).run().cpu().numpy()
+        want = fp64.rmsd(x, x, f, sel)
+        assert np.abs(got - want).max() < TOL_FP64
+        assert got[f] == 0.0
+    other = gen.walk(seed=22, n_frames=3, n_atoms=400)
+    rsel = np.sort(np.random.default_rng(3).choice(400, 220, replace=False))
+    got = plan.set_reference(other[1], rsel).run(self_frame=-1).cpu().numpy()
+    want = fp64.rmsd(x, other, 1, sel, rsel)
+    assert np.abs(got - want).max() < TOL_FP64
