@@ -511,6 +511,8 @@ def main():
     ap.add_argument("--frames-grid", type=int, default=0, choices=[0, 1, 2],
                     help="grid of the per-frame kernel: 1 equal quad counts, 2 one wave "
                          "per resident slot (0 = library default, 2)")
+    ap.add_argument("--frames-row-lanes", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
+                    help="lanes per frame of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
     per_workload = {"one-vs-all": (200, 100), "all-pairs": (5, 3), "batched": (50, 20),
                     "hbm-ceiling": (20, 10), "serving": (0, 0)}
@@ -532,6 +534,7 @@ def main():
 
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, args.frames_waves)
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_GRID, args.frames_grid)
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_ROW_LANES, args.frames_row_lanes)
     if args.workload == "hbm-ceiling":
         print(json.dumps(bench_hbm_ceiling(args)), flush=True)
         return

@@ -53,13 +53,18 @@ int hr_device_cu_count(void);
  *   HR_TUNE_FRAMES_WAVES_PER_SIMD  register budget of the per-frame kernel:
  *                                  3 (default, no spills) or 4 waves/SIMD
  *   HR_TUNE_FRAMES_GRID            grid of the per-frame kernel (whole
- *                                  frame quads per wave either way):
+ *                                  frame sets per wave either way):
  *                                  2 (default) one wave per resident slot;
  *                                  1 only as many waves as give every wave
- *                                  the same quad count */
+ *                                  the same set count
+ *   HR_TUNE_FRAMES_ROW_LANES       lanes per frame of the per-frame kernel:
+ *                                  0 (default) chosen by selection size,
+ *                                  or 4, 8, 16, 32, 64 (16 down to 1 frames
+ *                                  per wave) */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
-#define HR_TUNE_COUNT 2
+#define HR_TUNE_FRAMES_ROW_LANES 2
+#define HR_TUNE_COUNT 3
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 

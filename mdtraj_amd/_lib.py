@@ -16,6 +16,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hiprmsd.h")
 HR_OK = 0
 HR_TUNE_FRAMES_WAVES_PER_SIMD = 0
 HR_TUNE_FRAMES_GRID = 1
+HR_TUNE_FRAMES_ROW_LANES = 2
 
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1

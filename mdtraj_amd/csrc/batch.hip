@@ -78,20 +78,45 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     double* s_res = smem_d + wv * kWave * kNS;
+    float bx[kRingB][kItemAtoms], by[kRingB][kItemAtoms], bz[kRingB][kItemAtoms];
+    float qx[kRingB], qy[kRingB], qz[kRingB];
     unsigned u = 0;
     if (lane == 0) u = atomicAdd(ticket, 1u);
-    u = __shfl(u, 0, kWave);
+    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, kWave));
+    DevUnit unit{};
+    DevSystem s{};
+    if ((long long)u < n_units) {
+        unit = units[u];
+        s = sys[unit.system];
+        prime_ring_batch(s.xyz, unit.base, unit.cnt, s.n_atoms, s.ref_sel, s.ref_sel + s.n_atoms,
+                         s.ref_sel + 2 * s.n_atoms, lane, bx, by, bz, qx, qy, qz);
+    }
     while ((long long)u < n_units) {
         // take the next ticket now; its latency hides behind this unit
         unsigned next = 0;
         if (lane == 0) next = atomicAdd(ticket, 1u);
-        const DevUnit unit = units[u];
-        const DevSystem s = sys[unit.system];
         const int n = s.n_atoms;
-        const double rs[4] = {s.ref_stats[0], s.ref_stats[1], s.ref_stats[2], s.ref_stats[3]};
-        process_group_batch(s.xyz, unit.base, unit.cnt, n, s.ref_sel, s.ref_sel + n,
-                            s.ref_sel + 2 * n, rs, (long long)s.ref_frame, s.out, s_res, lane);
-        u = __shfl(next, 0, kWave);
+        run_group_batch(s.xyz, unit.base, unit.cnt, n, s.ref_sel, s.ref_sel + n,
+                        s.ref_sel + 2 * n, s_res, lane, bx, by, bz, qx, qy, qz);
+        // the ring is empty: start the next unit's loads before this one's
+        // QCP solves, so their latency hides behind them
+        const unsigned nu = __builtin_amdgcn_readfirstlane(__shfl(next, 0, kWave));
+        DevUnit unit_n{};
+        DevSystem s_n{};
+        if ((long long)nu < n_units) {
+            unit_n = units[nu];
+            s_n = sys[unit_n.system];
+            prime_ring_batch(s_n.xyz, unit_n.base, unit_n.cnt, s_n.n_atoms, s_n.ref_sel,
+                             s_n.ref_sel + s_n.n_atoms, s_n.ref_sel + 2 * s_n.n_atoms, lane, bx,
+                             by, bz, qx, qy, qz);
+        }
+        const GlobalDouble* st = global_ptr(s.ref_stats);
+        const double rs[4] = {st[0], st[1], st[2], st[3]};
+        finish_group<Kind::Rmsd>(unit.base, unit.cnt, (double)n, rs, (long long)s.ref_frame, s.out,
+                                 nullptr, nullptr, s_res, lane);
+        u = nu;
+        unit = unit_n;
+        s = s_n;
     }
 }
 

@@ -80,6 +80,11 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_FRAMES_GRID must be 0, 1 or 2 (got %d)", value);
         return HR_ERR_INVALID;
     }
+    if (knob == HR_TUNE_FRAMES_ROW_LANES && value != 0 && value != 4 && value != 8 &&
+        value != 16 && value != 32 && value != 64) {
+        hr::set_error("HR_TUNE_FRAMES_ROW_LANES must be 0, 4, 8, 16, 32 or 64 (got %d)", value);
+        return HR_ERR_INVALID;
+    }
     hr::g_tuning[knob].store(value);
     return HR_OK;
 }

@@ -56,6 +56,29 @@ inline int launch_status(const char* what) {
 // --------------------------------------------------------------- device side
 constexpr int kWave = 64;
 
+// Coordinates, references and indices always live in global memory, but a
+// pointer the kernel reads from a descriptor table is generic to the
+// compiler, which then emits flat loads.  Flat loads count against lgkmcnt as
+// well as vmcnt, so every LDS access or cross-lane permute that waits on
+// lgkmcnt would also drain the frame loads in flight.  Loads through these
+// casts are global_load instructions whatever the pointer's origin.
+typedef __attribute__((address_space(1))) const float GlobalFloat;
+typedef __attribute__((address_space(1))) const int GlobalInt;
+__device__ __forceinline__ const GlobalFloat* global_ptr(const float* p) {
+    return (const GlobalFloat*)p;
+}
+__device__ __forceinline__ const GlobalInt* global_ptr(const int* p) {
+    return (const GlobalInt*)p;
+}
+typedef __attribute__((address_space(1))) const double GlobalDouble;
+__device__ __forceinline__ const GlobalDouble* global_ptr(const double* p) {
+    return (const GlobalDouble*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global_out(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);

@@ -1,7 +1,7 @@
 // Device code shared by the per-frame kernels (one-vs-all, superpose fit,
-// no-superpose, batched systems): FP64 per-frame sums, the quarter-wave
-// (16 lanes per frame) item loop with its register ring, DPP row reductions,
-// and the per-frame finish (QCP or no-superpose) one lane per frame.
+// no-superpose, batched systems): FP64 per-frame sums, the row item loop
+// (kRL lanes per frame) with its register ring, DPP row reductions, and the
+// per-frame finish (QCP or no-superpose) one lane per frame.
 #pragma once
 
 #include "common.h"
@@ -181,38 +181,52 @@ constexpr size_t result_lds_bytes() {
     return (size_t)kWavesPerBlock * kWave * SumsPerFrame<kKind>::value * sizeof(double);
 }
 
-// Quarter-wave layout: 16 lanes own one frame, a wave works on 4 frames at
-// once (row r = lane / 16 holds frame base + r).  An item is (frame quad q,
-// atom block b): lane t of a row takes the selected atoms
-// b * 16 * kItemAtoms + t + 16 u, u < kItemAtoms.
+// Row layout: a row of kRL lanes (4, 8, 16, 32 or 64) owns one frame, so a
+// wave works on kWave / kRL frames at once (row r = lane / kRL holds frame
+// base + r).  An item is (frame set q, atom block b): lane t of a row takes
+// the selected atoms b * kRL * kItemAtoms + t + kRL u, u < kItemAtoms, so
+// one load instruction reads kRL consecutive selected atoms of each row's
+// frame.  kRowLanes is the batched kernel's row width.
 constexpr int kRowLanes = 16;
 constexpr int kItemAtoms = 4;   // atoms per lane per item
-constexpr int kRing = 4;        // items in the register ring (kRing - 1 in flight)
+constexpr int kRing = 3;        // items in the register ring (kRing - 1 in flight)
 
 // Frame coordinates are read once per launch: nontemporal loads (one
 // global_load_dwordx3 ... nt per atom) keep them from displacing reusable
 // lines in the caches; on MI355X they stream 2-5% faster than plain loads
 // (100,000 frames x 2,000 atoms: 0.419 ms against 0.426-0.442 ms).
-template <bool kAxis, bool kGather>
-__device__ __forceinline__ void load_item(const float* __restrict__ fr, bool row_ok,
-                                          int rowstride, const int* __restrict__ ix, int n_sel,
-                                          int t, int b, float (&ax)[kItemAtoms],
-                                          float (&ay)[kItemAtoms], float (&az)[kItemAtoms]) {
-    const int j0 = b * kRowLanes * kItemAtoms + t;
+// Slots past the selection (and rows past the group's last frame, for which
+// fr points at a valid frame) are zeroed where they are used (valid_atom).
+//   kBranchFree false: such slots skip their loads (a whole instruction when
+//     no lane needs it), the cheaper form for short frames.
+//   kBranchFree true: every slot loads, past-the-end ones the last selected
+//     atom.  The batched kernel needs this form: with its ring carried from
+//     one work unit to the next, a load under a branch is merged with the
+//     branch's zero by a register copy, and that copy makes the wave wait
+//     for each load as soon as it is issued (one load in flight, not two
+//     items).
+template <bool kAxis, bool kGather, int kRL = kRowLanes, bool kBranchFree = false>
+__device__ __forceinline__ void load_item(const float* __restrict__ fr, int rowstride,
+                                          const int* __restrict__ ix, int n_sel, int t, int b,
+                                          float (&ax)[kItemAtoms], float (&ay)[kItemAtoms],
+                                          float (&az)[kItemAtoms]) {
+    const int j0 = b * kRL * kItemAtoms + t;
+    const GlobalFloat* g = global_ptr(fr);
 #pragma unroll
     for (int u = 0; u < kItemAtoms; ++u) {
-        const int j = j0 + u * kRowLanes;
+        const int jr = j0 + u * kRL;
+        const int j = kBranchFree ? min(jr, n_sel - 1) : jr;
         float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (row_ok && j < n_sel) {
+        if (kBranchFree || j < n_sel) {
             if constexpr (kAxis) {
-                x = __builtin_nontemporal_load(fr + j);
-                y = __builtin_nontemporal_load(fr + rowstride + j);
-                z = __builtin_nontemporal_load(fr + 2 * rowstride + j);
+                x = __builtin_nontemporal_load(g + j);
+                y = __builtin_nontemporal_load(g + rowstride + j);
+                z = __builtin_nontemporal_load(g + 2 * rowstride + j);
             } else {
                 const int a = kGather ? ix[j] : j;
-                x = __builtin_nontemporal_load(fr + 3 * a);
-                y = __builtin_nontemporal_load(fr + 3 * a + 1);
-                z = __builtin_nontemporal_load(fr + 3 * a + 2);
+                x = __builtin_nontemporal_load(g + 3 * a);
+                y = __builtin_nontemporal_load(g + 3 * a + 1);
+                z = __builtin_nontemporal_load(g + 3 * a + 2);
             }
         }
         ax[u] = x;
@@ -221,15 +235,34 @@ __device__ __forceinline__ void load_item(const float* __restrict__ fr, bool row
     }
 }
 
-template <Kind kKind>
+// Coordinates of a ring slot as accumulated: zero for slots past the
+// selection or past the group's last frame, so they add nothing.
+__device__ __forceinline__ float valid_atom(bool ok, float v) { return ok ? v : 0.0f; }
+
+// Sum over each row of kRL lanes (4, 8, 16, 32 or 64), result in every lane
+// of the row: the DPP steps of row16_sum up to the row width (xor 1, xor 2,
+// half-row mirror, row mirror), then lane-xor shuffles across rows of 16.
+template <int kRL>
+__device__ __forceinline__ double row_sum(double v) {
+    static_assert(kRL == 4 || kRL == 8 || kRL == 16 || kRL == 32 || kRL == 64, "row width");
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    if constexpr (kRL >= 8) v += dpp_move<0x141>(v);
+    if constexpr (kRL >= 16) v += dpp_move<0x140>(v);
+    if constexpr (kRL >= 32) v += __shfl_xor(v, 16, kWave);
+    if constexpr (kRL >= 64) v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
+template <Kind kKind, int kRL = kRowLanes>
 __device__ __forceinline__ void row_reduce(Sums& s) {
-    s.q = row16_sum(s.q);
+    s.q = row_sum<kRL>(s.q);
     if constexpr (kKind != Kind::NoSuperpose) {
-        s.sx = row16_sum(s.sx);
-        s.sy = row16_sum(s.sy);
-        s.sz = row16_sum(s.sz);
+        s.sx = row_sum<kRL>(s.sx);
+        s.sy = row_sum<kRL>(s.sy);
+        s.sz = row_sum<kRL>(s.sz);
 #pragma unroll
-        for (int e = 0; e < 9; ++e) s.p[e] = row16_sum(s.p[e]);
+        for (int e = 0; e < 9; ++e) s.p[e] = row_sum<kRL>(s.p[e]);
     }
 }
 
@@ -251,24 +284,24 @@ __device__ __forceinline__ void finish_group(long long base, int cnt, double n,
         Sums mine;
         load_sums<kKind>(mine, s_res + lane * kNS);
         if constexpr (kKind == Kind::NoSuperpose) {
-            out[f] = (float)sqrt(mine.q / n);
+            global_out(out)[f] = (float)sqrt(mine.q / n);
         } else {
             double M[9], ga, gb;
             centred(mine, rs, n, M, ga, gb);
             if constexpr (kKind == Kind::Rmsd) {
                 const double msd = qcp_msd(M, ga, gb, n);
-                out[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
+                global_out(out)[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
             } else {
                 double R[9];
                 const double lam = qcp_solve<true>(M, 0.5 * (ga + gb), R);
 #pragma unroll
-                for (int e = 0; e < 9; ++e) rot_out[9 * f + e] = (float)R[e];
-                mu_out[3 * f + 0] = mine.sx / n;
-                mu_out[3 * f + 1] = mine.sy / n;
-                mu_out[3 * f + 2] = mine.sz / n;
+                for (int e = 0; e < 9; ++e) global_out(rot_out)[9 * f + e] = (float)R[e];
+                global_out(mu_out)[3 * f + 0] = mine.sx / n;
+                global_out(mu_out)[3 * f + 1] = mine.sy / n;
+                global_out(mu_out)[3 * f + 2] = mine.sz / n;
                 if (out) {
                     const double msd = (ga + gb - 2.0 * lam) / n;
-                    out[f] = (float)sqrt(msd > 0.0 ? msd : 0.0);
+                    global_out(out)[f] = (float)sqrt(msd > 0.0 ? msd : 0.0);
                 }
             }
         }
@@ -279,73 +312,111 @@ __device__ __forceinline__ void finish_group(long long base, int cnt, double n,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One group of up to 64 consecutive frames (base .. base + cnt) of one
-// trajectory through the wave: items (frame quad, atom block) stream through
-// the register ring, each quad's row sums are reduced (DPP) and parked in the
-// wave's LDS slots, then one lane per frame finishes (QCP or no-superpose).
-template <Kind kKind, bool kAxis, bool kGather>
-__device__ __forceinline__ void process_group(
-    const float* __restrict__ xyz, long long base, int cnt, long long frame_stride,
+// Loads of the first kRing - 1 items of a group (base .. base + cnt) into
+// ring slots 0 .. kRing - 2.
+template <bool kAxis, bool kGather, int kRL>
+__device__ __forceinline__ void prime_ring(const float* __restrict__ xyz, long long base, int cnt,
+                                           long long frame_stride, int rowstride,
+                                           const int* __restrict__ ix, int n_sel, int n_blk,
+                                           int row, int t, float (&bx)[kRing][kItemAtoms],
+                                           float (&by)[kRing][kItemAtoms],
+                                           float (&bz)[kRing][kItemAtoms]) {
+    constexpr int kRows = kWave / kRL;
+    const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+#pragma unroll
+    for (int r = 0; r < kRing - 1; ++r) {
+        if (r < n_items) {
+            const int q = r / n_blk, b = r - q * n_blk;
+            const int k = kRows * q + row;
+            load_item<kAxis, kGather, kRL>(xyz + (base + (k < cnt ? k : 0)) * frame_stride,
+                                           rowstride, ix, n_sel, t, b, bx[r], by[r], bz[r]);
+        }
+    }
+}
+
+// The wave's frames f_begin .. f_end of one trajectory, in groups of up to
+// 64: items (frame set, atom block) stream through the register ring, each
+// set's row sums are reduced (DPP) and parked in the wave's LDS slots, and
+// at the end of a group one lane per frame finishes (QCP or no-superpose).
+// The ring is empty when a group's last item has been accumulated, so the
+// next group's first kRing - 1 items are loaded before the finish: their
+// latency hides behind the QCP solves instead of following them.  The fit
+// finish (rotation and centroid) needs the registers those loads would hold,
+// so that kind primes the ring after its finish.
+template <Kind kKind, bool kAxis, bool kGather, int kRL>
+__device__ __forceinline__ void process_chunk(
+    const float* __restrict__ xyz, long long f_begin, long long f_end, long long frame_stride,
     int rowstride, const int* __restrict__ ix, int n_sel, const float* __restrict__ rx,
     const float* __restrict__ ry, const float* __restrict__ rz, const double (&rs)[4],
     long long self_frame, float* __restrict__ out, float* __restrict__ rot_out,
     double* __restrict__ mu_out, double* __restrict__ s_res, int lane) {
     constexpr int kNS = SumsPerFrame<kKind>::value;
-    const int row = lane / kRowLanes, t = lane % kRowLanes;
+    constexpr int kRows = kWave / kRL;  // frames side by side in the wave
+    const int row = lane / kRL, t = lane % kRL;
     const double n = (double)n_sel;
-    const int n_blk = (n_sel + kRowLanes * kItemAtoms - 1) / (kRowLanes * kItemAtoms);
-    const int n_quads = (cnt + 3) / 4;
-    const int n_items = n_quads * n_blk;
-    // item -> (quad, block); loads of `item` into ring slot `slot`
+    const int n_blk = (n_sel + kRL * kItemAtoms - 1) / (kRL * kItemAtoms);
     float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
+    if (f_begin < f_end)
+        prime_ring<kAxis, kGather, kRL>(xyz, f_begin, (int)min((long long)kWave, f_end - f_begin),
+                                        frame_stride, rowstride, ix, n_sel, n_blk, row, t, bx, by,
+                                        bz);
+    for (long long base = f_begin; base < f_end; base += kWave) {
+        const int cnt = (int)min((long long)kWave, f_end - base);
+        const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+        Sums s = {};
+        for (int it0 = 0; it0 < n_items; it0 += kRing) {
 #pragma unroll
-    for (int r = 0; r < kRing - 1; ++r) {
-        if (r < n_items) {
-            const int q = r / n_blk, b = r - q * n_blk;
-            const int k = 4 * q + row;
-            load_item<kAxis, kGather>(xyz + (base + k) * frame_stride, k < cnt, rowstride, ix,
-                                      n_sel, t, b, bx[r], by[r], bz[r]);
-        }
-    }
-    Sums s = {};
-    for (int it0 = 0; it0 < n_items; it0 += kRing) {
-#pragma unroll
-        for (int r = 0; r < kRing; ++r) {
-            const int it = it0 + r;
-            if (it < n_items) {
-                // refill the slot freed one step ago with item it + kRing - 1
-                const int nxt = it + kRing - 1;
-                if (nxt < n_items) {
-                    const int q = nxt / n_blk, b = nxt - q * n_blk;
-                    const int k = 4 * q + row;
-                    const int slot = (r + kRing - 1) % kRing;  // static once unrolled
-                    load_item<kAxis, kGather>(xyz + (base + k) * frame_stride, k < cnt,
-                                              rowstride, ix, n_sel, t, b, bx[slot], by[slot],
-                                              bz[slot]);
-                }
-                const int q = it / n_blk, b = it - q * n_blk;
-                const int j0 = b * kRowLanes * kItemAtoms + t;
-#pragma unroll
-                for (int u = 0; u < kItemAtoms; ++u) {
-                    const int j = j0 + u * kRowLanes;
-                    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-                    if (j < n_sel) {
-                        ux = rx[j];
-                        uy = ry[j];
-                        uz = rz[j];
+            for (int r = 0; r < kRing; ++r) {
+                const int it = it0 + r;
+                if (it < n_items) {
+                    // refill the slot freed one step ago with item it + kRing - 1
+                    const int nxt = it + kRing - 1;
+                    if (nxt < n_items) {
+                        const int q = nxt / n_blk, b = nxt - q * n_blk;
+                        const int k = kRows * q + row;
+                        const int slot = (r + kRing - 1) % kRing;  // static once unrolled
+                        load_item<kAxis, kGather, kRL>(
+                            xyz + (base + (k < cnt ? k : 0)) * frame_stride, rowstride, ix,
+                            n_sel, t, b, bx[slot], by[slot], bz[slot]);
                     }
-                    accumulate<kKind>(s, bx[r][u], by[r][u], bz[r][u], ux, uy, uz);
-                }
-                if (b == n_blk - 1) {
-                    row_reduce<kKind>(s);
-                    const int k = 4 * q + row;
-                    if (t == 0 && k < cnt) store_sums<kKind>(s_res + k * kNS, s);
-                    s = Sums{};
+                    const int q = it / n_blk, b = it - q * n_blk;
+                    const int j0 = b * kRL * kItemAtoms + t;
+                    const bool row_ok = kRows * q + row < cnt;
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int j = j0 + u * kRL;
+                        const bool ok = row_ok && j < n_sel;
+                        const int jc = min(j, n_sel - 1);
+                        // products with a zeroed frame atom vanish; the
+                        // no-superpose difference needs the reference zeroed
+                        constexpr bool kRef = kKind == Kind::NoSuperpose;
+                        accumulate<kKind>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                          valid_atom(ok, bz[r][u]),
+                                          kRef ? valid_atom(ok, rx[jc]) : rx[jc],
+                                          kRef ? valid_atom(ok, ry[jc]) : ry[jc],
+                                          kRef ? valid_atom(ok, rz[jc]) : rz[jc]);
+                    }
+                    if (b == n_blk - 1) {
+                        row_reduce<kKind, kRL>(s);
+                        const int k = kRows * q + row;
+                        if (t == 0 && k < cnt) store_sums<kKind>(s_res + k * kNS, s);
+                        s = Sums{};
+                    }
                 }
             }
         }
+        constexpr bool kEarly = kKind != Kind::Fit;
+        const long long next = base + kWave;
+        if (kEarly && next < f_end)
+            prime_ring<kAxis, kGather, kRL>(xyz, next, (int)min((long long)kWave, f_end - next),
+                                            frame_stride, rowstride, ix, n_sel, n_blk, row, t,
+                                            bx, by, bz);
+        finish_group<kKind>(base, cnt, n, rs, self_frame, out, rot_out, mu_out, s_res, lane);
+        if (!kEarly && next < f_end)
+            prime_ring<kAxis, kGather, kRL>(xyz, next, (int)min((long long)kWave, f_end - next),
+                                            frame_stride, rowstride, ix, n_sel, n_blk, row, t,
+                                            bx, by, bz);
     }
-    finish_group<kKind>(base, cnt, n, rs, self_frame, out, rot_out, mu_out, s_res, lane);
 }
 
 // Batched variant of the item loop (one system's frames, all atoms, no
@@ -353,10 +424,10 @@ __device__ __forceinline__ void process_group(
 // staged in LDS up front: every item's block of 64 reference atoms rides in
 // the load ring with the frame data (lane l loads atom 64 b + l of the three
 // planes, 2 items ahead) and each lane fetches the four atoms it uses from
-// the lanes that loaded them.  Same arithmetic as process_group.
+// the lanes that loaded them.  Same arithmetic as process_chunk.
 constexpr int kRingB = 3;
 
-__device__ __forceinline__ void load_item_batch(const float* __restrict__ fr, bool row_ok,
+__device__ __forceinline__ void load_item_batch(const float* __restrict__ fr,
                                                 int n, int t, int b, int lane,
                                                 const float* __restrict__ rx,
                                                 const float* __restrict__ ry,
@@ -364,41 +435,51 @@ __device__ __forceinline__ void load_item_batch(const float* __restrict__ fr, bo
                                                 float (&ax)[kItemAtoms], float (&ay)[kItemAtoms],
                                                 float (&az)[kItemAtoms], float& qx, float& qy,
                                                 float& qz) {
-    load_item<false, false>(fr, row_ok, 0, nullptr, n, t, b, ax, ay, az);
-    const int jr = b * kRowLanes * kItemAtoms + lane;
-    qx = 0.0f;
-    qy = 0.0f;
-    qz = 0.0f;
-    if (jr < n) {
-        qx = rx[jr];
-        qy = ry[jr];
-        qz = rz[jr];
-    }
+    load_item<false, false, kRowLanes, true>(fr, 0, nullptr, n, t, b, ax, ay, az);
+    const int jr = min(b * kRowLanes * kItemAtoms + lane, n - 1);
+    qx = global_ptr(rx)[jr];
+    qy = global_ptr(ry)[jr];
+    qz = global_ptr(rz)[jr];
 }
 
-__device__ __forceinline__ void process_group_batch(
+// Loads of the first kRingB - 1 items of a batched group into ring slots
+// 0 .. kRingB - 2.
+__device__ __forceinline__ void prime_ring_batch(
     const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
     const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
-    const double (&rs)[4], long long self_frame, float* __restrict__ out,
-    double* __restrict__ s_res, int lane) {
-    constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
-    constexpr int kBlk = kRowLanes * kItemAtoms;  // atoms per item block
+    int lane, float (&bx)[kRingB][kItemAtoms], float (&by)[kRingB][kItemAtoms],
+    float (&bz)[kRingB][kItemAtoms], float (&qx)[kRingB], float (&qy)[kRingB],
+    float (&qz)[kRingB]) {
+    constexpr int kBlk = kRowLanes * kItemAtoms;
     const int row = lane / kRowLanes, t = lane % kRowLanes;
     const long long frame_stride = 3LL * n_atoms;
     const int n_blk = (n_atoms + kBlk - 1) / kBlk;
-    const int n_quads = (cnt + 3) / 4;
-    const int n_items = n_quads * n_blk;
-    float bx[kRingB][kItemAtoms], by[kRingB][kItemAtoms], bz[kRingB][kItemAtoms];
-    float qx[kRingB], qy[kRingB], qz[kRingB];
+    const int n_items = (cnt + 3) / 4 * n_blk;
 #pragma unroll
     for (int r = 0; r < kRingB - 1; ++r) {
         if (r < n_items) {
             const int q = r / n_blk, b = r - q * n_blk;
             const int k = 4 * q + row;
-            load_item_batch(xyz + (base + k) * frame_stride, k < cnt, n_atoms, t, b, lane, rx,
+            load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b, lane, rx,
                             ry, rz, bx[r], by[r], bz[r], qx[r], qy[r], qz[r]);
         }
     }
+}
+
+// The items of one batched group (ring primed by prime_ring_batch); on
+// return the group's row sums are parked in s_res and the ring is empty.
+__device__ __forceinline__ void run_group_batch(
+    const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
+    const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
+    double* __restrict__ s_res, int lane, float (&bx)[kRingB][kItemAtoms],
+    float (&by)[kRingB][kItemAtoms], float (&bz)[kRingB][kItemAtoms], float (&qx)[kRingB],
+    float (&qy)[kRingB], float (&qz)[kRingB]) {
+    constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
+    constexpr int kBlk = kRowLanes * kItemAtoms;  // atoms per item block
+    const int row = lane / kRowLanes, t = lane % kRowLanes;
+    const long long frame_stride = 3LL * n_atoms;
+    const int n_blk = (n_atoms + kBlk - 1) / kBlk;
+    const int n_items = (cnt + 3) / 4 * n_blk;
     Sums s = {};
     for (int it0 = 0; it0 < n_items; it0 += kRingB) {
 #pragma unroll
@@ -410,19 +491,21 @@ __device__ __forceinline__ void process_group_batch(
                     const int q = nxt / n_blk, b = nxt - q * n_blk;
                     const int k = 4 * q + row;
                     const int slot = (r + kRingB - 1) % kRingB;
-                    load_item_batch(xyz + (base + k) * frame_stride, k < cnt, n_atoms, t, b, lane,
+                    load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b, lane,
                                     rx, ry, rz, bx[slot], by[slot], bz[slot], qx[slot], qy[slot],
                                     qz[slot]);
                 }
                 // atom 64 b + t + 16 u of the reference was loaded by lane
                 // 16 u + t: fetch it from there (cross-lane permute)
                 const int q = it / n_blk, b = it - q * n_blk;
+                const bool row_ok = 4 * q + row < cnt;
 #pragma unroll
                 for (int u = 0; u < kItemAtoms; ++u) {
                     const int src = u * kRowLanes + t;
-                    accumulate<Kind::Rmsd>(s, bx[r][u], by[r][u], bz[r][u],
-                                           __shfl(qx[r], src, kWave), __shfl(qy[r], src, kWave),
-                                           __shfl(qz[r], src, kWave));
+                    const bool ok = row_ok && b * kBlk + src < n_atoms;
+                    accumulate<Kind::Rmsd>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                           valid_atom(ok, bz[r][u]), __shfl(qx[r], src, kWave),
+                                           __shfl(qy[r], src, kWave), __shfl(qz[r], src, kWave));
                 }
                 if (b == n_blk - 1) {
                     row_reduce<Kind::Rmsd>(s);
@@ -433,8 +516,6 @@ __device__ __forceinline__ void process_group_batch(
             }
         }
     }
-    finish_group<Kind::Rmsd>(base, cnt, (double)n_atoms, rs, self_frame, out, nullptr, nullptr,
-                             s_res, lane);
 }
 
 

@@ -1,9 +1,9 @@
 // One-vs-all RMSD, centering, no-superpose RMSD and superposition kernels
 // (gfx950).  Layout in HBM: coordinates stay as the caller holds them
 // (atom-major (F, N, 3) float32 or axis-major (F, 3, rowstride)); nothing is
-// re-laid-out.  One wavefront owns one frame at a time and sweeps its selected
-// atoms with lane-strided loads; per-frame sums are accumulated in FP64 in
-// registers and reduced across the wave with cross-lane shuffles; the QCP
+// re-laid-out.  A row of 4 - 64 lanes of a wavefront owns one frame at a time
+// and sweeps its selected atoms with lane-strided loads; per-frame sums are
+// accumulated in FP64 in registers and reduced across the row; the QCP
 // solve then runs one lane per frame (lane k finishes the k-th frame of the
 // wave's current group of 64).  The reference selection and the atom indices
 // are staged once per workgroup in LDS.
@@ -13,16 +13,17 @@ namespace hr {
 namespace {
 
 // Per-frame kernel.  Each wave owns a contiguous chunk of frames and walks
-// its items (frame quad, atom block) through a ring of kRing register
+// its items (frame set, atom block) through a ring of kRing register
 // buffers, so the loads of the next kRing - 1 items are in flight while one
-// is accumulated in FP64.  At the end of a quad each row's sums are reduced
-// across its 16 lanes (DPP, VALU only) and parked in LDS; every 64 frames
-// the QCP solve runs one lane per frame.  The grid is one workgroup per
-// resident slot (occupancy API), so no block waits for a second round.
+// is accumulated in FP64.  A frame set is kWave / kRL frames, one per row of
+// kRL lanes.  At the end of a set each row's sums are reduced across its
+// lanes (DPP within 16 lanes, shuffles across) and parked in LDS; every 64
+// frames the QCP solve runs one lane per frame.  The grid is one workgroup
+// per resident slot (occupancy API), so no block waits for a second round.
 //   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
 //   kGather atom-major target with an index list
 //   kLds    reference selection and indices staged in LDS (else read via L2)
-template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves>
+template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves, int kRL>
 __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
     const float* __restrict__ xyz, long long n_frames, long long frame_stride, int rowstride,
     const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
@@ -67,19 +68,17 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
         rs[2] = ref_stats[2];
         rs[3] = ref_stats[3];
     }
-    // whole frame quads per wave (a quad fills the wave's four rows)
+    // whole frame sets per wave (a set fills the wave's kWave / kRL rows)
+    constexpr int kRows = kWave / kRL;
     const long long wave = (long long)blockIdx.x * kWavesPerBlock + wv;
     const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
-    const long long n_quads = (n_frames + 3) / 4;
-    const long long f_begin = 4 * (wave * n_quads / n_waves);
-    const long long f_end = min(n_frames, 4 * ((wave + 1) * n_quads / n_waves));
+    const long long n_sets = (n_frames + kRows - 1) / kRows;
+    const long long f_begin = kRows * (wave * n_sets / n_waves);
+    const long long f_end = min(n_frames, kRows * ((wave + 1) * n_sets / n_waves));
 
-    for (long long base = f_begin; base < f_end; base += kWave) {
-        const int cnt = (int)min((long long)kWave, f_end - base);
-        process_group<kKind, kAxis, kGather>(xyz, base, cnt, frame_stride, rowstride, ix, n_sel,
-                                             rx, ry, rz, rs, self_frame, out, rot_out, mu_out,
-                                             s_res, lane);
-    }
+    process_chunk<kKind, kAxis, kGather, kRL>(xyz, f_begin, f_end, frame_stride, rowstride, ix,
+                                              n_sel, rx, ry, rz, rs, self_frame, out, rot_out,
+                                              mu_out, s_res, lane);
 }
 
 // In-place centering and traces, one wave per frame (two passes over the
@@ -187,38 +186,53 @@ int resident_blocks(Kernel kernel, size_t lds, long long n_waves_wanted, int* bl
     return HR_OK;
 }
 
-// Grid of the per-frame kernel; the kernel gives each wave whole frame
-// quads.  Default: one wave per resident slot.  HR_TUNE_FRAMES_GRID = 1: with
-// Q quads and S slots, k = ceil(Q / S) quads per wave needs only ceil(Q / k)
-// waves, and launching that many gives every wave k or k - 1 quads.  On
-// MI355X the full grid is faster (100,000 frames x 2,000 atoms: 0.432 ms
-// against 0.449 ms): more waves keep more loads in flight, which outweighs
-// the uneven last quad.
+// Grid of the per-frame kernel; the kernel gives each wave whole frame sets
+// (`rows` frames, one per row).  Default: one wave per resident slot.
+// HR_TUNE_FRAMES_GRID = 1: with Q sets and S slots, k = ceil(Q / S) sets per
+// wave needs only ceil(Q / k) waves, and launching that many gives every wave
+// k or k - 1 sets.  On MI355X the full grid is faster (100,000 frames x 2,000
+// atoms, four frames per wave: 0.432 ms against 0.449 ms): more waves keep
+// more loads in flight, which outweighs the uneven last set.
 template <typename Kernel>
-int frames_grid(Kernel kernel, size_t lds, long long n_frames, int* blocks) {
-    const long long n_quads = (n_frames + 3) / 4;
-    int rc = resident_blocks(kernel, lds, n_quads, blocks);
+int frames_grid(Kernel kernel, size_t lds, long long n_frames, int rows, int* blocks) {
+    const long long n_sets = (n_frames + rows - 1) / rows;
+    int rc = resident_blocks(kernel, lds, n_sets, blocks);
     if (rc || tuning(HR_TUNE_FRAMES_GRID) != 1) return rc;
     const long long slots = (long long)*blocks * kWavesPerBlock;
-    const long long per_wave = (n_quads + slots - 1) / slots;
-    const long long waves = (n_quads + per_wave - 1) / per_wave;
+    const long long per_wave = (n_sets + slots - 1) / slots;
+    const long long waves = (n_sets + per_wave - 1) / per_wave;
     const long long b = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
     *blocks = (int)(b < 1 ? 1 : b);
     return HR_OK;
 }
 
-template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves>
+template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves, int kRL>
 int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
                     long long frame_stride, int rowstride, const int* idx, int n_sel,
                     const float* ref_sel, const double* ref_stats, long long self_frame,
                     float* out, float* rot_out, double* mu_out) {
     int blocks = 0, rc;
-    auto k = frames_kernel<kKind, kAxis, kGather, kLds, kMinWaves>;
-    if ((rc = frames_grid(k, l.lds, n_frames, &blocks))) return rc;
+    auto k = frames_kernel<kKind, kAxis, kGather, kLds, kMinWaves, kRL>;
+    if ((rc = frames_grid(k, l.lds, n_frames, kWave / kRL, &blocks))) return rc;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
                        rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
                        mu_out);
     return launch_status("frames_kernel");
+}
+
+// Lanes per frame (row width of the per-frame kernel) by selection size,
+// from a sweep over 8 - 3,000 selected atoms at ~2.4 GB per launch on MI355X:
+// narrow rows keep the lanes of short frames busy and amortise the per-frame
+// reduction over more frames; wide rows make each load instruction one
+// contiguous span of a frame (768 bytes at 64 lanes) instead of several short
+// ones.  Gathered selections of 1,400+ atoms stream best one frame per wave,
+// contiguous ones two.  HR_TUNE_FRAMES_ROW_LANES overrides the choice.
+int auto_row_lanes(int n_sel, bool gather) {
+    if (n_sel < 28) return 4;
+    if (n_sel < 112) return 8;
+    if (n_sel < 720) return 16;
+    if (n_sel < 1400 || !gather) return 32;
+    return 64;
 }
 
 // Register budget of the per-frame kernel: 3 waves per SIMD (no spills) or 4
@@ -229,13 +243,19 @@ int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_
                   const float* ref_sel, const double* ref_stats, long long self_frame, float* out,
                   float* rot_out, double* mu_out) {
     const bool four = tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4;
+    const int rl = tuning(HR_TUNE_FRAMES_ROW_LANES) ? tuning(HR_TUNE_FRAMES_ROW_LANES)
+                                                   : auto_row_lanes(n_sel, kGather);
 #define HR_FRAMES_ARGS l, s, xyz, n_frames, frame_stride, rowstride, idx, n_sel, ref_sel, \
                        ref_stats, self_frame, out, rot_out, mu_out
-    if (l.use_lds)
-        return four ? launch_frames_v<kKind, kAxis, kGather, true, 4>(HR_FRAMES_ARGS)
-                    : launch_frames_v<kKind, kAxis, kGather, true, 3>(HR_FRAMES_ARGS);
-    return four ? launch_frames_v<kKind, kAxis, kGather, false, 4>(HR_FRAMES_ARGS)
-                : launch_frames_v<kKind, kAxis, kGather, false, 3>(HR_FRAMES_ARGS);
+#define HR_FRAMES_RL(LDS, W)                                                               \
+    (rl == 64   ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 64>(HR_FRAMES_ARGS)       \
+     : rl == 32 ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 32>(HR_FRAMES_ARGS)       \
+     : rl == 16 ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 16>(HR_FRAMES_ARGS)       \
+     : rl == 8  ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 8>(HR_FRAMES_ARGS)        \
+                : launch_frames_v<kKind, kAxis, kGather, LDS, W, 4>(HR_FRAMES_ARGS))
+    if (l.use_lds) return four ? HR_FRAMES_RL(true, 4) : HR_FRAMES_RL(true, 3);
+    return four ? HR_FRAMES_RL(false, 4) : HR_FRAMES_RL(false, 3);
+#undef HR_FRAMES_RL
 #undef HR_FRAMES_ARGS
 }
 

@@ -142,6 +142,33 @@ def test_shapes_vs_fp64(gpu, n_frames, n_atoms, n_sel):
     assert np.abs(got - o).max() < TOL_FP64
 
 
+@pytest.mark.parametrize("row_lanes", [4, 8, 16, 32, 64])
+def test_row_lanes_layouts_vs_fp64(gpu, row_lanes):
+    """Every lanes-per-frame layout of the per-frame kernel (16, 8, 4, 2 or
+    1 frames per wave): frame counts not a multiple of the frames per wave,
+    selections shorter than one row's atom block, gathered and contiguous
+    atoms, the no-superpose and fit (superpose) kinds."""
+    from mdtraj_amd import _lib
+
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_ROW_LANES, row_lanes)
+    try:
+        for n_frames, n_atoms, n_sel in [(131, 300, 220), (67, 2000, None), (5, 40, 17),
+                                         (70, 1001, None)]:
+            x = gen.walk(seed=n_frames * 7 + n_atoms, n_frames=n_frames, n_atoms=n_atoms)
+            idx = None if n_sel is None else gen.subset(n_atoms, n_atoms, n_sel)
+            f = n_frames // 3
+            t = T(x)
+            got = md.rmsd(t, t, f, atom_indices=idx)
+            assert np.abs(got - fp64.rmsd(x, x, f, idx)).max() < TOL_FP64
+            assert got[f] == 0.0
+            ns = md.rmsd(t, t, f, atom_indices=idx, superpose=False)
+            assert np.abs(ns - fp64.rmsd_nosuperpose(x, x, f, idx)).max() < TOL_FP64
+            sup = T(x.copy()).superpose(t, f, atom_indices=idx)
+            assert np.abs(sup.xyz - fp64.superpose(x, x, f, idx)).max() < 2e-5
+    finally:
+        _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_ROW_LANES, 0)
+
+
 def test_ref_atom_indices_mapping(gpu):
     # tests/test_rmsd.py:143-171: swapping halves with ref_atom_indices
     rng = np.random.default_rng(52)
