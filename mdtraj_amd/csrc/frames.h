@@ -19,6 +19,11 @@ constexpr size_t kRefOffset = 256;
 
 enum class Kind { Rmsd, NoSuperpose, Fit };
 
+// What a per-frame workgroup stages in LDS (see plan in one_vs_all.hip).
+constexpr int kStageNone = 0;
+constexpr int kStageIdx = 1;
+constexpr int kStageAll = 2;
+
 // Gather the reference selection into axis-major planes and sum it in FP64
 // (one workgroup of kBlock threads).
 __device__ __forceinline__ void ref_prep_body(const float* __restrict__ ref,
@@ -235,6 +240,32 @@ __device__ __forceinline__ void load_item(const float* __restrict__ fr, int rows
     }
 }
 
+// Reference atoms of the same slots as load_item, for a reference that is
+// read from global memory (not staged in LDS): loaded with the frame data so
+// that one wait covers both (a reference read at accumulation time would
+// wait for every frame load issued before it).
+template <int kRL>
+__device__ __forceinline__ void load_ref_item(const float* __restrict__ rx,
+                                              const float* __restrict__ ry,
+                                              const float* __restrict__ rz, int n_sel, int t,
+                                              int b, float (&qx)[kItemAtoms],
+                                              float (&qy)[kItemAtoms], float (&qz)[kItemAtoms]) {
+    const int j0 = b * kRL * kItemAtoms + t;
+#pragma unroll
+    for (int u = 0; u < kItemAtoms; ++u) {
+        const int j = j0 + u * kRL;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (j < n_sel) {
+            x = global_ptr(rx)[j];
+            y = global_ptr(ry)[j];
+            z = global_ptr(rz)[j];
+        }
+        qx[u] = x;
+        qy[u] = y;
+        qz[u] = z;
+    }
+}
+
 // Coordinates of a ring slot as accumulated: zero for slots past the
 // selection or past the group's last frame, so they add nothing.
 __device__ __forceinline__ float valid_atom(bool ok, float v) { return ok ? v : 0.0f; }
@@ -313,14 +344,15 @@ __device__ __forceinline__ void finish_group(long long base, int cnt, double n,
 }
 
 // Loads of the first kRing - 1 items of a group (base .. base + cnt) into
-// ring slots 0 .. kRing - 2.
-template <bool kAxis, bool kGather, int kRL>
-__device__ __forceinline__ void prime_ring(const float* __restrict__ xyz, long long base, int cnt,
-                                           long long frame_stride, int rowstride,
-                                           const int* __restrict__ ix, int n_sel, int n_blk,
-                                           int row, int t, float (&bx)[kRing][kItemAtoms],
-                                           float (&by)[kRing][kItemAtoms],
-                                           float (&bz)[kRing][kItemAtoms]) {
+// ring slots 0 .. kRing - 2 (with their reference atoms when kRefRing).
+template <bool kAxis, bool kGather, int kRL, bool kRefRing>
+__device__ __forceinline__ void prime_ring(
+    const float* __restrict__ xyz, long long base, int cnt, long long frame_stride,
+    int rowstride, const int* __restrict__ ix, int n_sel, const float* __restrict__ rx,
+    const float* __restrict__ ry, const float* __restrict__ rz, int n_blk, int row, int t,
+    float (&bx)[kRing][kItemAtoms], float (&by)[kRing][kItemAtoms],
+    float (&bz)[kRing][kItemAtoms], float (&qx)[kRing][kItemAtoms],
+    float (&qy)[kRing][kItemAtoms], float (&qz)[kRing][kItemAtoms]) {
     constexpr int kRows = kWave / kRL;
     const int n_items = (cnt + kRows - 1) / kRows * n_blk;
 #pragma unroll
@@ -330,6 +362,7 @@ __device__ __forceinline__ void prime_ring(const float* __restrict__ xyz, long l
             const int k = kRows * q + row;
             load_item<kAxis, kGather, kRL>(xyz + (base + (k < cnt ? k : 0)) * frame_stride,
                                            rowstride, ix, n_sel, t, b, bx[r], by[r], bz[r]);
+            if constexpr (kRefRing) load_ref_item<kRL>(rx, ry, rz, n_sel, t, b, qx[r], qy[r], qz[r]);
         }
     }
 }
@@ -342,8 +375,11 @@ __device__ __forceinline__ void prime_ring(const float* __restrict__ xyz, long l
 // next group's first kRing - 1 items are loaded before the finish: their
 // latency hides behind the QCP solves instead of following them.  The fit
 // finish (rotation and centroid) needs the registers those loads would hold,
-// so that kind primes the ring after its finish.
-template <Kind kKind, bool kAxis, bool kGather, int kRL>
+// so that kind primes the ring after its finish.  kRefRing: the reference is
+// in global memory and rides the ring with the frame data; a ring that large
+// would not fit beside the QCP solves either, and it serves selections too
+// large for LDS, whose long frames make the early start matter little.
+template <Kind kKind, bool kAxis, bool kGather, int kRL, bool kRefRing>
 __device__ __forceinline__ void process_chunk(
     const float* __restrict__ xyz, long long f_begin, long long f_end, long long frame_stride,
     int rowstride, const int* __restrict__ ix, int n_sel, const float* __restrict__ rx,
@@ -356,10 +392,11 @@ __device__ __forceinline__ void process_chunk(
     const double n = (double)n_sel;
     const int n_blk = (n_sel + kRL * kItemAtoms - 1) / (kRL * kItemAtoms);
     float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
+    float qx[kRing][kItemAtoms], qy[kRing][kItemAtoms], qz[kRing][kItemAtoms];  // kRefRing
     if (f_begin < f_end)
-        prime_ring<kAxis, kGather, kRL>(xyz, f_begin, (int)min((long long)kWave, f_end - f_begin),
-                                        frame_stride, rowstride, ix, n_sel, n_blk, row, t, bx, by,
-                                        bz);
+        prime_ring<kAxis, kGather, kRL, kRefRing>(
+            xyz, f_begin, (int)min((long long)kWave, f_end - f_begin), frame_stride, rowstride, ix,
+            n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
     for (long long base = f_begin; base < f_end; base += kWave) {
         const int cnt = (int)min((long long)kWave, f_end - base);
         const int n_items = (cnt + kRows - 1) / kRows * n_blk;
@@ -378,6 +415,9 @@ __device__ __forceinline__ void process_chunk(
                         load_item<kAxis, kGather, kRL>(
                             xyz + (base + (k < cnt ? k : 0)) * frame_stride, rowstride, ix,
                             n_sel, t, b, bx[slot], by[slot], bz[slot]);
+                        if constexpr (kRefRing)
+                            load_ref_item<kRL>(rx, ry, rz, n_sel, t, b, qx[slot], qy[slot],
+                                               qz[slot]);
                     }
                     const int q = it / n_blk, b = it - q * n_blk;
                     const int j0 = b * kRL * kItemAtoms + t;
@@ -390,11 +430,13 @@ __device__ __forceinline__ void process_chunk(
                         // products with a zeroed frame atom vanish; the
                         // no-superpose difference needs the reference zeroed
                         constexpr bool kRef = kKind == Kind::NoSuperpose;
+                        const float ux = kRefRing ? qx[r][u] : rx[jc];
+                        const float uy = kRefRing ? qy[r][u] : ry[jc];
+                        const float uz = kRefRing ? qz[r][u] : rz[jc];
                         accumulate<kKind>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
-                                          valid_atom(ok, bz[r][u]),
-                                          kRef ? valid_atom(ok, rx[jc]) : rx[jc],
-                                          kRef ? valid_atom(ok, ry[jc]) : ry[jc],
-                                          kRef ? valid_atom(ok, rz[jc]) : rz[jc]);
+                                          valid_atom(ok, bz[r][u]), kRef ? valid_atom(ok, ux) : ux,
+                                          kRef ? valid_atom(ok, uy) : uy,
+                                          kRef ? valid_atom(ok, uz) : uz);
                     }
                     if (b == n_blk - 1) {
                         row_reduce<kKind, kRL>(s);
@@ -405,17 +447,17 @@ __device__ __forceinline__ void process_chunk(
                 }
             }
         }
-        constexpr bool kEarly = kKind != Kind::Fit;
+        constexpr bool kEarly = kKind != Kind::Fit && !kRefRing;
         const long long next = base + kWave;
         if (kEarly && next < f_end)
-            prime_ring<kAxis, kGather, kRL>(xyz, next, (int)min((long long)kWave, f_end - next),
-                                            frame_stride, rowstride, ix, n_sel, n_blk, row, t,
-                                            bx, by, bz);
+            prime_ring<kAxis, kGather, kRL, kRefRing>(
+                xyz, next, (int)min((long long)kWave, f_end - next), frame_stride, rowstride, ix,
+                n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
         finish_group<kKind>(base, cnt, n, rs, self_frame, out, rot_out, mu_out, s_res, lane);
         if (!kEarly && next < f_end)
-            prime_ring<kAxis, kGather, kRL>(xyz, next, (int)min((long long)kWave, f_end - next),
-                                            frame_stride, rowstride, ix, n_sel, n_blk, row, t,
-                                            bx, by, bz);
+            prime_ring<kAxis, kGather, kRL, kRefRing>(
+                xyz, next, (int)min((long long)kWave, f_end - next), frame_stride, rowstride, ix,
+                n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
     }
 }
 

@@ -22,8 +22,11 @@ namespace {
 // per resident slot (occupancy API), so no block waits for a second round.
 //   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
 //   kGather atom-major target with an index list
-//   kLds    reference selection and indices staged in LDS (else read via L2)
-template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves, int kRL>
+//   kStage  what the workgroup stages in LDS: kStageAll the reference
+//           selection and the indices; kStageIdx the indices only (a
+//           gathered selection whose reference does not fit; the reference
+//           then rides the load ring); kStageNone nothing
+template <Kind kKind, bool kAxis, bool kGather, int kStage, int kMinWaves, int kRL>
 __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
     const float* __restrict__ xyz, long long n_frames, long long frame_stride, int rowstride,
     const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
@@ -35,31 +38,31 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
     const int wv = threadIdx.x / kWave;
     double* s_res = smem_d + wv * kWave * kNS;
     float* smem = reinterpret_cast<float*>(smem_d + kWavesPerBlock * kWave * kNS);
-    const float* rx;
-    const float* ry;
-    const float* rz;
-    const int* ix;
-    if constexpr (kLds) {
+    const float* rx = ref_sel;
+    const float* ry = ref_sel + n_sel;
+    const float* rz = ref_sel + 2 * n_sel;
+    const int* ix = idx;
+    if constexpr (kStage != kStageNone) {
+        // LDS after the result area: [reference planes (kStageAll)] [indices]
         float* s_rx = smem;
         float* s_ry = smem + n_sel;
         float* s_rz = smem + 2 * n_sel;
-        int* s_idx = reinterpret_cast<int*>(smem + 3 * n_sel);
+        int* s_idx = reinterpret_cast<int*>(kStage == kStageAll ? smem + 3 * n_sel : smem);
         for (int j = threadIdx.x; j < n_sel; j += kBlock) {
-            s_rx[j] = ref_sel[j];
-            s_ry[j] = ref_sel[n_sel + j];
-            s_rz[j] = ref_sel[2 * n_sel + j];
+            if constexpr (kStage == kStageAll) {
+                s_rx[j] = ref_sel[j];
+                s_ry[j] = ref_sel[n_sel + j];
+                s_rz[j] = ref_sel[2 * n_sel + j];
+            }
             if constexpr (kGather) s_idx[j] = idx[j];
         }
         __syncthreads();
-        rx = s_rx;
-        ry = s_ry;
-        rz = s_rz;
-        ix = s_idx;
-    } else {
-        rx = ref_sel;
-        ry = ref_sel + n_sel;
-        rz = ref_sel + 2 * n_sel;
-        ix = idx;
+        if constexpr (kStage == kStageAll) {
+            rx = s_rx;
+            ry = s_ry;
+            rz = s_rz;
+        }
+        if constexpr (kGather) ix = s_idx;
     }
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
     if constexpr (kKind != Kind::NoSuperpose) {
@@ -76,9 +79,9 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
     const long long f_begin = kRows * (wave * n_sets / n_waves);
     const long long f_end = min(n_frames, kRows * ((wave + 1) * n_sets / n_waves));
 
-    process_chunk<kKind, kAxis, kGather, kRL>(xyz, f_begin, f_end, frame_stride, rowstride, ix,
-                                              n_sel, rx, ry, rz, rs, self_frame, out, rot_out,
-                                              mu_out, s_res, lane);
+    process_chunk<kKind, kAxis, kGather, kRL, kStage != kStageAll>(
+        xyz, f_begin, f_end, frame_stride, rowstride, ix, n_sel, rx, ry, rz, rs, self_frame, out,
+        rot_out, mu_out, s_res, lane);
 }
 
 // In-place centering and traces, one wave per frame (two passes over the
@@ -153,20 +156,31 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(float* __restrict__ xyz, 
 
 struct Launch {
     size_t lds;
-    bool use_lds;
+    int stage;  // kStageAll, kStageIdx or kStageNone
 };
 
-// LDS plan: the result area always, the reference selection (and indices)
-// when they fit in 64 KB together with it.
+// LDS plan: the result area always; the reference selection and the indices
+// when they fit in 64 KB together with it, else the indices alone (gathered
+// selections), else nothing.  64 KB keeps three workgroups (12 waves) per CU
+// for selections up to ~2,400 contiguous or ~1,900 gathered atoms.
 template <Kind kKind>
 int plan(int n_sel, bool gather, Launch* l) {
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
     const size_t res = result_lds_bytes<kKind>();
-    const size_t need = (size_t)n_sel * (3 * sizeof(float) + (gather ? sizeof(int) : 0));
-    size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
-    l->use_lds = res + need <= cap;
-    l->lds = res + (l->use_lds ? need : 0);
+    const size_t ref = (size_t)n_sel * 3 * sizeof(float);
+    const size_t ids = gather ? (size_t)n_sel * sizeof(int) : 0;
+    const size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
+    if (res + ref + ids <= cap) {
+        l->stage = kStageAll;
+        l->lds = res + ref + ids;
+    } else if (gather && res + ids <= cap) {
+        l->stage = kStageIdx;
+        l->lds = res + ids;
+    } else {
+        l->stage = kStageNone;
+        l->lds = res;
+    }
     return HR_OK;
 }
 
@@ -206,13 +220,13 @@ int frames_grid(Kernel kernel, size_t lds, long long n_frames, int rows, int* bl
     return HR_OK;
 }
 
-template <Kind kKind, bool kAxis, bool kGather, bool kLds, int kMinWaves, int kRL>
+template <Kind kKind, bool kAxis, bool kGather, int kStage, int kMinWaves, int kRL>
 int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long n_frames,
                     long long frame_stride, int rowstride, const int* idx, int n_sel,
                     const float* ref_sel, const double* ref_stats, long long self_frame,
                     float* out, float* rot_out, double* mu_out) {
     int blocks = 0, rc;
-    auto k = frames_kernel<kKind, kAxis, kGather, kLds, kMinWaves, kRL>;
+    auto k = frames_kernel<kKind, kAxis, kGather, kStage, kMinWaves, kRL>;
     if ((rc = frames_grid(k, l.lds, n_frames, kWave / kRL, &blocks))) return rc;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
                        rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
@@ -253,8 +267,13 @@ int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_
      : rl == 16 ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 16>(HR_FRAMES_ARGS)       \
      : rl == 8  ? launch_frames_v<kKind, kAxis, kGather, LDS, W, 8>(HR_FRAMES_ARGS)        \
                 : launch_frames_v<kKind, kAxis, kGather, LDS, W, 4>(HR_FRAMES_ARGS))
-    if (l.use_lds) return four ? HR_FRAMES_RL(true, 4) : HR_FRAMES_RL(true, 3);
-    return four ? HR_FRAMES_RL(false, 4) : HR_FRAMES_RL(false, 3);
+    if (l.stage == kStageAll)
+        return four ? HR_FRAMES_RL(kStageAll, 4) : HR_FRAMES_RL(kStageAll, 3);
+    if constexpr (kGather) {
+        if (l.stage == kStageIdx)
+            return four ? HR_FRAMES_RL(kStageIdx, 4) : HR_FRAMES_RL(kStageIdx, 3);
+    }
+    return four ? HR_FRAMES_RL(kStageNone, 4) : HR_FRAMES_RL(kStageNone, 3);
 #undef HR_FRAMES_RL
 #undef HR_FRAMES_ARGS
 }
