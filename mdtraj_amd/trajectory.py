@@ -142,9 +142,11 @@ class Trajectory:
         lib = load()
         rxyz = reference.xyz if hasattr(reference, "xyz") else reference
         on_device = self.is_on_device
+        # In place, as the reference (core/trajectory.py:1128 aliases
+        # self.xyz): a device-resident float32 trajectory is transformed in
+        # its own memory; the reference frame is gathered into the workspace
+        # before any frame is moved, so superposing onto itself is safe.
         X = dev.to_device_f32(self._xyz)
-        if X is self._xyz:
-            X = X.clone()  # the result is assigned back through the setter
         R = dev.to_device_f32(rxyz[frame])
         n_frames, n_atoms = int(X.shape[0]), int(X.shape[1])
         n_sel = n_atoms if atom_indices is None else len(atom_indices)
@@ -169,5 +171,15 @@ class Trajectory:
                     "your trajectory and (2) divide and/or multiply your `_xyz` dataset by "
                     "multiples of 10 before running superpose again. Then, (3) revert that "
                     "multiplication/division after the calculations.")
-        self.xyz = X if on_device else dev.to_host(X)
+        if on_device:
+            self.xyz = X
+        else:
+            out = dev.to_host(X)
+            a = self._xyz
+            if (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous
+                    and a.flags.writeable):
+                a[...] = out  # the caller's array, as the reference modifies it
+                self._rmsd_traces = None
+            else:
+                self.xyz = out
         return self

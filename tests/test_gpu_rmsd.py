@@ -268,6 +268,26 @@ def test_superpose_semantics(gpu):
     assert np.all(np.isfinite(t.xyz))
 
 
+def test_superpose_in_place(gpu):
+    """The reference transforms the trajectory's own array
+    (core/trajectory.py:1128 aliases self.xyz): a host array held elsewhere
+    sees the result, and a device-resident trajectory keeps its memory."""
+    import torch
+
+    w = gen.walk(seed=9, n_frames=40, n_atoms=60)
+    want = fp64.superpose(w, w, 3)
+    a = w.copy()
+    t = T(a)
+    t.superpose(t, 3)
+    assert t.xyz is a
+    assert np.abs(a - want).max() < 2e-5
+    d = torch.from_numpy(w.copy()).cuda()
+    td = T(d)
+    td.superpose(td, 3)
+    assert td.xyz.data_ptr() == d.data_ptr()
+    assert np.abs(d.cpu().numpy() - want).max() < 2e-5
+
+
 def test_superpose_refinds_swap(gpu):
     # tests/test_rmsd.py:143-171
     rng = np.random.default_rng(52)
