@@ -60,11 +60,19 @@ int hr_device_cu_count(void);
  *   HR_TUNE_FRAMES_ROW_LANES       lanes per frame of the per-frame kernel:
  *                                  0 (default) chosen by selection size,
  *                                  or 4, 8, 16, 32, 64 (16 down to 1 frames
- *                                  per wave) */
+ *                                  per wave)
+ *   HR_TUNE_BATCH_SCHEDULE         work split of batch plans created after
+ *                                  the call: 1 (default) work units largest
+ *                                  first, each wave's first unit its wave
+ *                                  index and the rest from an atomic ticket;
+ *                                  2 a static split of all systems' frames
+ *                                  into one contiguous, cost-balanced range
+ *                                  per wave (no tile shape to tune) */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2
-#define HR_TUNE_COUNT 3
+#define HR_TUNE_BATCH_SCHEDULE 3
+#define HR_TUNE_COUNT 4
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 

@@ -39,6 +39,8 @@ class BatchRMSD:
             raise ValueError("no systems")
         self.ref_frames = [0] * n if ref_frames is None else [int(r) for r in ref_frames]
         self.out = [dev.empty_f32(max(int(x.shape[0]), 1)) for x in self.xyz]
+        # per-system result views, made once: run() only launches the graph
+        self._views = [o[: int(x.shape[0])] for o, x in zip(self.out, self.xyz)]
         if frames_per_unit is None or waves is None or unit_kib is None:
             from .autotune import tuned_shape
 
@@ -80,8 +82,10 @@ class BatchRMSD:
         return sum(12 * int(x.shape[0]) * int(x.shape[1]) for x in self.xyz)
 
     def run(self):
+        """One graph launch on torch's current stream; returns the per-system
+        float32 device views of the results (the same tensors every call)."""
         call("hr_batch_plan_launch", self._plan, dev.stream())
-        return [o[: int(x.shape[0])] for o, x in zip(self.out, self.xyz)]
+        return list(self._views)
 
     def close(self):
         if self._plan is not None and self._plan.value:

@@ -7,15 +7,24 @@
 //   1. a one-thread fill kernel that zeroes the unit ticket (no memset node),
 //   2. reference prep: one workgroup per system gathers its reference frame
 //      into axis-major planes and sums it in FP64,
-//   3. the persistent scoring kernel: one workgroup per resident slot; each
-//      wave takes the next unit from the ticket (vector atomic, the next one
-//      requested while the current unit runs) until the table is exhausted,
-//      and runs the per-frame code on it (16 lanes per frame, FP64 sums, the
-//      reference streamed through the load ring with the frames and passed
-//      between lanes by permutes, QCP one lane per frame).
-// Every wave leaves when the ticket passes the table's end, so the grid
+//   3. the persistent scoring kernel: one workgroup per resident slot; wave w
+//      takes unit w first and then the next unit from the ticket (vector
+//      atomic, the next one requested while the current unit runs) until the
+//      table is exhausted, and runs the per-frame code on it (16 lanes per
+//      frame, FP64 sums, the reference streamed through the load ring with
+//      the frames and passed between lanes by permutes, QCP one lane per
+//      frame).  Taking the first units without the ticket matters: when
+//      every wave of the grid asks one counter at launch, the atomics to that
+//      address serialise and the last waves start a third of a millisecond
+//      late (100,000 x 2,000 in 33-frame units: 0.77 ms against 0.49 ms).
+// Every wave leaves when its unit index passes the table's end, so the grid
 // drains on every launch.  A plan must not be launched on two streams at once
 // (the ticket is shared).
+//
+// HR_TUNE_BATCH_SCHEDULE = 2 replaces the ticket by a static split (nodes 2
+// and 3 only): the frames of all systems form one sequence and each wave
+// takes one contiguous, cost-balanced range of it, like the one-vs-all
+// kernel on a single trajectory, the reference riding the load ring.
 #include <algorithm>
 #include <vector>
 
@@ -29,6 +38,9 @@ struct hr_batch_plan {
     void* d_systems = nullptr;
     void* d_units = nullptr;
     unsigned* d_ticket = nullptr;
+    long long* d_prefix = nullptr;  // static split: cost before system s, s = 0..n
+    long long total_cost = 0;
+    int schedule = 1;               // 1 ticket, 2 static split
     char* d_ref = nullptr;
     int blocks = 0;
     size_t lds = 0;
@@ -80,9 +92,12 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     double* s_res = smem_d + wv * kWave * kNS;
     float bx[kRingB][kItemAtoms], by[kRingB][kItemAtoms], bz[kRingB][kItemAtoms];
     float qx[kRingB], qy[kRingB], qz[kRingB];
-    unsigned u = 0;
-    if (lane == 0) u = atomicAdd(ticket, 1u);
-    u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, kWave));
+    // The first unit of every wave is its wave index, the rest come from the
+    // ticket, which therefore counts from the grid's wave count: thousands of
+    // waves taking their first unit from one atomic counter at launch would
+    // start one after another as the atomics to that address serialise.
+    const unsigned n_waves = gridDim.x * kWavesPerBlock;
+    unsigned u = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     DevUnit unit{};
     DevSystem s{};
     if ((long long)u < n_units) {
@@ -100,7 +115,7 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
                         s.ref_sel + 2 * n, s_res, lane, bx, by, bz, qx, qy, qz);
         // the ring is empty: start the next unit's loads before this one's
         // QCP solves, so their latency hides behind them
-        const unsigned nu = __builtin_amdgcn_readfirstlane(__shfl(next, 0, kWave));
+        const unsigned nu = n_waves + __builtin_amdgcn_readfirstlane(__shfl(next, 0, kWave));
         DevUnit unit_n{};
         DevSystem s_n{};
         if ((long long)nu < n_units) {
@@ -120,6 +135,74 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     }
 }
 
+// Cost of one frame in the static split: its 12 N bytes plus the per-frame
+// work that does not scale with N (row reduction, QCP), which on MI355X costs
+// about as much as streaming 240 more bytes (one-vs-all sweep: ~40 ps per
+// frame + ~2 ps per atom).
+constexpr long long kFrameCost = 240;
+
+__host__ __device__ __forceinline__ long long frame_cost(int n_atoms) {
+    return 12LL * n_atoms + kFrameCost;
+}
+
+// One system's frames fa .. fb through the one-vs-all item loop, the row
+// width chosen from its atom count, the reference read from global memory
+// with the frame data (it is not staged in LDS: a wave's range can span
+// several systems).
+__device__ __forceinline__ void static_segment(const DevSystem& d, long long fa, long long fb,
+                                               double* __restrict__ s_res, int lane) {
+    const int n = d.n_atoms;
+    const GlobalDouble* st = global_ptr(d.ref_stats);
+    const double rs[4] = {st[0], st[1], st[2], st[3]};
+#define HR_SEGMENT(RL)                                                                       \
+    process_chunk<Kind::Rmsd, false, false, RL, true>(d.xyz, fa, fb, 3LL * n, 0, nullptr, n,  \
+                                                      d.ref_sel, d.ref_sel + n,              \
+                                                      d.ref_sel + 2 * n, rs,                 \
+                                                      (long long)d.ref_frame, d.out, nullptr, \
+                                                      nullptr, s_res, lane)
+    switch (auto_row_lanes(n, false)) {
+        case 4: HR_SEGMENT(4); break;
+        case 8: HR_SEGMENT(8); break;
+        case 16: HR_SEGMENT(16); break;
+        default: HR_SEGMENT(32); break;
+    }
+#undef HR_SEGMENT
+}
+
+// Static split: the frames of all systems, in order, form one sequence whose
+// frame (s, f) starts at cost prefix[s] + f * frame_cost(N_s); wave w of W
+// owns the frames starting in [C w / W, C (w + 1) / W), C the total cost -
+// one contiguous range per wave, as the one-vs-all kernel splits a single
+// trajectory, balanced by bytes plus per-frame work, with no atomics.
+template <int kMinWaves>
+__global__ __launch_bounds__(kBlock, kMinWaves) void batch_static_kernel(
+    const DevSystem* __restrict__ sys, const long long* __restrict__ prefix, int n_systems,
+    long long total) {
+    extern __shared__ double smem_d[];
+    constexpr int kNS = SumsPerFrame<Kind::Rmsd>::value;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    double* s_res = smem_d + wv * kWave * kNS;
+    const long long wave = (long long)blockIdx.x * kWavesPerBlock + wv;
+    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
+    const long long c0 = total / n_waves * wave + total % n_waves * wave / n_waves;
+    const long long c1 = total / n_waves * (wave + 1) + total % n_waves * (wave + 1) / n_waves;
+    // last system starting at or before c0
+    int lo = 0, hi = n_systems;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (prefix[mid] <= c0) lo = mid;
+        else hi = mid;
+    }
+    for (int si = lo; si < n_systems && prefix[si] < c1; ++si) {
+        const DevSystem d = sys[si];
+        const long long pfc = frame_cost(d.n_atoms);
+        const long long fa = min(d.n_frames, c0 > prefix[si] ? (c0 - prefix[si] + pfc - 1) / pfc : 0);
+        const long long fb = min(d.n_frames, (c1 - prefix[si] + pfc - 1) / pfc);
+        if (fa < fb) static_segment(d, fa, fb, s_res, lane);
+    }
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 void release(hr_batch_plan* p) {
@@ -130,34 +213,43 @@ void release(hr_batch_plan* p) {
     if (p->d_systems) (void)hipFree(p->d_systems);
     if (p->d_units) (void)hipFree(p->d_units);
     if (p->d_ticket) (void)hipFree(p->d_ticket);
+    if (p->d_prefix) (void)hipFree(p->d_prefix);
     if (p->d_ref) (void)hipFree(p->d_ref);
     delete p;
 }
 
 template <int kMinWaves>
 int capture_plan(hr_batch_plan* p) {
-    auto k = batch_kernel<kMinWaves>;
+    const bool fixed = p->schedule == 2;
+    const void* k = fixed ? reinterpret_cast<const void*>(batch_static_kernel<kMinWaves>)
+                          : reinterpret_cast<const void*>(batch_kernel<kMinWaves>);
     const size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
                        sizeof(double);
     int per_cu = 0;
-    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k), kBlock, lds));
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
     long long blocks = (long long)(per_cu < 1 ? 1 : per_cu) * d->cu_count;
-    const long long by_work = (p->n_units + kWavesPerBlock - 1) / kWavesPerBlock;
+    // ticket: no more waves than units; static split: no more than frames
+    const long long work = fixed ? p->total_frames : p->n_units;
+    const long long by_work = (work + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks > by_work) blocks = by_work;
     if (blocks < 1) blocks = 1;
     p->blocks = (int)blocks;
     p->lds = lds;
     hipStream_t s = p->capture;
     HR_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
+    if (!fixed) hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
     hipLaunchKernelGGL(batch_ref_prep_kernel, dim3(p->n_systems), dim3(kBlock), 0, s,
                        static_cast<const DevSystem*>(p->d_systems));
-    hipLaunchKernelGGL(k, dim3(p->blocks), dim3(kBlock), lds, s,
-                       static_cast<const DevSystem*>(p->d_systems),
-                       static_cast<const DevUnit*>(p->d_units), p->n_units, p->d_ticket);
+    if (fixed)
+        hipLaunchKernelGGL(batch_static_kernel<kMinWaves>, dim3(p->blocks), dim3(kBlock), lds, s,
+                           static_cast<const DevSystem*>(p->d_systems), p->d_prefix,
+                           p->n_systems, p->total_cost);
+    else
+        hipLaunchKernelGGL(batch_kernel<kMinWaves>, dim3(p->blocks), dim3(kBlock), lds, s,
+                           static_cast<const DevSystem*>(p->d_systems),
+                           static_cast<const DevUnit*>(p->d_units), p->n_units, p->d_ticket);
     hipError_t launch = hipGetLastError();
     hipError_t end = hipStreamEndCapture(s, &p->graph);
     if (launch != hipSuccess || end != hipSuccess) {
@@ -205,6 +297,12 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
     p->device = d->device;
     p->n_systems = n_systems;
     p->frames_per_unit = frames_per_unit;
+    p->schedule = tuning(HR_TUNE_BATCH_SCHEDULE) == 2 ? 2 : 1;
+    // static split: cost of the frames before each system
+    std::vector<long long> prefix(n_systems + 1, 0);
+    for (int i = 0; i < n_systems; ++i)
+        prefix[i + 1] = prefix[i] + systems[i].n_frames * frame_cost(systems[i].n_atoms);
+    p->total_cost = prefix[n_systems];
 
     // reference workspace: per system planes (3 N floats) then 4 doubles
     std::vector<size_t> ref_off(n_systems);
@@ -260,6 +358,9 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
     HR_PLAN_TRY(hipMalloc(&p->d_systems, sizeof(DevSystem) * n_systems));
     HR_PLAN_TRY(hipMalloc(&p->d_units, sizeof(DevUnit) * (units.empty() ? 1 : units.size())));
     HR_PLAN_TRY(hipMalloc(&p->d_ticket, 256));
+    HR_PLAN_TRY(hipMalloc(&p->d_prefix, sizeof(long long) * (n_systems + 1)));
+    HR_PLAN_TRY(hipMemcpy(p->d_prefix, prefix.data(), sizeof(long long) * (n_systems + 1),
+                          hipMemcpyHostToDevice));
     HR_PLAN_TRY(hipMemcpy(p->d_systems, ds.data(), sizeof(DevSystem) * n_systems,
                           hipMemcpyHostToDevice));
     if (!units.empty())

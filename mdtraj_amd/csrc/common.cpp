@@ -85,6 +85,10 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_FRAMES_ROW_LANES must be 0, 4, 8, 16, 32 or 64 (got %d)", value);
         return HR_ERR_INVALID;
     }
+    if (knob == HR_TUNE_BATCH_SCHEDULE && value != 0 && value != 1 && value != 2) {
+        hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0, 1 or 2 (got %d)", value);
+        return HR_ERR_INVALID;
+    }
     hr::g_tuning[knob].store(value);
     return HR_OK;
 }

@@ -19,6 +19,21 @@ constexpr size_t kRefOffset = 256;
 
 enum class Kind { Rmsd, NoSuperpose, Fit };
 
+// Lanes per frame (row width of the per-frame item loop) by selection size,
+// from a sweep over 8 - 3,000 selected atoms at ~2.4 GB per launch on MI355X:
+// narrow rows keep the lanes of short frames busy and amortise the per-frame
+// reduction over more frames; wide rows make each load instruction one
+// contiguous span of a frame (768 bytes at 64 lanes) instead of several short
+// ones.  Gathered selections of 1,400+ atoms stream best one frame per wave,
+// contiguous ones two.
+__host__ __device__ __forceinline__ int auto_row_lanes(int n_sel, bool gather) {
+    if (n_sel < 28) return 4;
+    if (n_sel < 112) return 8;
+    if (n_sel < 720) return 16;
+    if (n_sel < 1400 || !gather) return 32;
+    return 64;
+}
+
 // What a per-frame workgroup stages in LDS (see plan in one_vs_all.hip).
 constexpr int kStageNone = 0;
 constexpr int kStageIdx = 1;

@@ -234,21 +234,8 @@ int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long 
     return launch_status("frames_kernel");
 }
 
-// Lanes per frame (row width of the per-frame kernel) by selection size,
-// from a sweep over 8 - 3,000 selected atoms at ~2.4 GB per launch on MI355X:
-// narrow rows keep the lanes of short frames busy and amortise the per-frame
-// reduction over more frames; wide rows make each load instruction one
-// contiguous span of a frame (768 bytes at 64 lanes) instead of several short
-// ones.  Gathered selections of 1,400+ atoms stream best one frame per wave,
-// contiguous ones two.  HR_TUNE_FRAMES_ROW_LANES overrides the choice.
-int auto_row_lanes(int n_sel, bool gather) {
-    if (n_sel < 28) return 4;
-    if (n_sel < 112) return 8;
-    if (n_sel < 720) return 16;
-    if (n_sel < 1400 || !gather) return 32;
-    return 64;
-}
-
+// auto_row_lanes (frames.h) picks the row width; HR_TUNE_FRAMES_ROW_LANES
+// overrides it.
 // Register budget of the per-frame kernel: 3 waves per SIMD (no spills) or 4
 // (more loads in flight, a few spilled registers); HR_TUNE_FRAMES_WAVES_PER_SIMD.
 template <Kind kKind, bool kAxis, bool kGather>

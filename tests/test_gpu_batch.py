@@ -22,12 +22,21 @@ def systems(seed=0, n=24):
     return out
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
 @pytest.mark.parametrize("fpu,kib,waves", [(64, 0, 3), (16, 0, 3), (7, 0, 4), (64, 0, 4),
                                            (64, 8, 3), (8, 64, 3)])
-def test_batch_vs_fp64(gpu, fpu, kib, waves):
+def test_batch_vs_fp64(gpu, fpu, kib, waves, schedule):
+    """Both work splits (ticket units, static cost split) over systems of
+    1 - 1,300 atoms and 1 - 139 frames."""
+    from mdtraj_amd import _lib
+
     xs = systems()
     refs = [int(np.random.default_rng(i).integers(0, x.shape[0])) for i, x in enumerate(xs)]
-    b = BatchRMSD(xs, refs, frames_per_unit=fpu, unit_kib=kib, waves=waves)
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_BATCH_SCHEDULE, schedule)
+    try:
+        b = BatchRMSD(xs, refs, frames_per_unit=fpu, unit_kib=kib, waves=waves)
+    finally:
+        _lib.call("hr_set_tuning", _lib.HR_TUNE_BATCH_SCHEDULE, 0)
     res = [r.cpu().numpy() for r in b.run()]
     for x, r, got in zip(xs, refs, res):
         want = fp64.rmsd(x, x, r)
