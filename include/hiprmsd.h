@@ -290,6 +290,39 @@ int hr_batch_plan_destroy(hr_batch_plan* plan);
 int64_t hr_batch_plan_units(const hr_batch_plan* plan);
 
 /* ---------------------------------------------------------------------------
+ * XTC ingestion: trajectory files decoded on the device
+ *
+ * Replaces the reference's frame loop over read_xtc
+ * (formats/xtc/xtc.pyx:424-517 -> formats/xtc/src/xdrfile_xtc.c:41-90, the
+ * header and box, and formats/xtc/src/xdrfile.c:778-1000,
+ * xdrfile_decompress_coord_float): the file's bytes are copied to HBM once
+ * and every frame is decoded in parallel into float32 (F, N, 3).
+ * ------------------------------------------------------------------------- */
+/* Per-frame status words of hr_xtc_decode. */
+#define HR_XTC_OK 0
+#define HR_XTC_BAD_MAGIC 1  /* not an XTC frame header */
+#define HR_XTC_ATOMS 2      /* atom count differs from n_atoms */
+#define HR_XTC_TRUNCATED 3  /* the frame's bytes end early */
+#define HR_XTC_STREAM 4     /* malformed bit stream (the reference would misread) */
+
+/* Frame index of XTC bytes in HOST memory: walks the frame headers
+ * (xdrfile_xtc.c:41-90), writes the byte offset of the first max_frames
+ * frames (offsets may be NULL with max_frames 0 to count them), and returns
+ * the frame count and the atom count.  A file that is not XTC, changes its
+ * atom count or ends inside a frame fails with HR_ERR_INVALID. */
+int hr_xtc_scan(const void* data, int64_t n_bytes, int64_t* offsets, int64_t max_frames,
+                int64_t* n_frames, int32_t* n_atoms);
+/* Decode n_frames frames of XTC bytes in DEVICE memory (4-byte aligned)
+ * starting at the device offsets[] (any subset of the frames, e.g. a
+ * stride): xyz (n_frames, n_atoms, 3), time, step, box (n_frames, 3, 3),
+ * precision (0 for uncompressed frames of <= 9 atoms) and status
+ * (n_frames HR_XTC_* words), all device arrays.  Results are bit-identical
+ * to the reference's reader. */
+int hr_xtc_decode(const void* data, int64_t n_bytes, const int64_t* offsets, int64_t n_frames,
+                  int32_t n_atoms, float* xyz, float* time, int32_t* step, float* box,
+                  float* prec, int32_t* status, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device
  * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
  * achieved rate is the HBM read ceiling the RMSD kernels are compared with.

@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(_HERE, "libhiprmsd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "hiprmsd.h")
 
 HR_OK = 0
+HR_ERR_INVALID = 1
+HR_ERR_HIP = 2
+HR_ERR_UNSUPPORTED = 3
 HR_TUNE_FRAMES_WAVES_PER_SIMD = 0
 HR_TUNE_FRAMES_GRID = 1
 HR_TUNE_FRAMES_ROW_LANES = 2
@@ -61,6 +64,8 @@ _SIGNATURES = {
     "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "hr_batch_plan_units": (_i64, [_vp]),
     "hr_hbm_read_probe": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
+    "hr_xtc_scan": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    "hr_xtc_decode": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()
