@@ -493,6 +493,109 @@ def bench_hbm_ceiling(args):
                          "unit": "GB/s", "frac": best[1] / HBM_PEAK_GBS, "traffic": None}}
 
 
+def bench_xtc_decode(args):
+    """XTC ingestion: decode every frame of a file resident in HBM into
+    float32 (F, N, 3).  Input: the 30-frame, 600-atom water fixture
+    (tests/golden/xtc_water.xtc, written by the reference's encoder)
+    repeated to 90,000 frames; frames are self-contained, so the repeated
+    file is a valid trajectory.  Timed: hr_xtc_decode alone (HIP events),
+    bytes already in HBM."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev
+    from mdtraj_amd.xtc import XTCTrajectoryFile
+
+    src = open(os.path.join(ROOT, "tests", "golden", "xtc_water.xtc"), "rb").read()
+    reps = 3000
+    fd, fn = tempfile.mkstemp(suffix=".xtc")
+    with os.fdopen(fd, "wb") as fh:
+        for _ in range(reps):
+            fh.write(src)
+    try:
+        f = XTCTrajectoryFile(fn)
+        F, N = len(f), f.n_atoms
+        d = dev.device()
+        xyz = torch.empty((F, N, 3), dtype=torch.float32, device=d)
+        tm = torch.empty(F, device=d)
+        st = torch.empty(F, dtype=torch.int32, device=d)
+        bx = torch.empty((F, 9), device=d)
+        pr = torch.empty(F, device=d)
+        status = torch.empty(F, dtype=torch.int32, device=d)
+        data = f._device_bytes()
+        offs = torch.from_numpy(f._offsets).to(d)
+
+        def run():
+            _lib.call("hr_xtc_decode", data.data_ptr(), data.numel(), offs.data_ptr(), F, N,
+                      xyz.data_ptr(), tm.data_ptr(), st.data_ptr(), bx.data_ptr(),
+                      pr.data_ptr(), status.data_ptr(), dev.stream())
+
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(args.steps):
+            run()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.steps
+        assert int(status.max().item()) == 0, "decode reported a malformed frame"
+        comp = data.numel()
+        out_bytes = 12 * F * N
+        achieved = (comp + out_bytes) / (ms * 1e-3) / 1e9
+        log(f"xtc decode {F} frames x {N} atoms: {ms:.3f} ms, {F / ms * 1e3:.3e} frames/s, "
+            f"{out_bytes / (ms * 1e-3) / 1e9:.0f} GB/s decoded")
+        res = {"metric": "XTC frames decoded/s (bit-identical to the reference reader)",
+               "value": F / (ms * 1e-3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+               "data": "the reference-encoded 600-atom water fixture repeated",
+               "config": {"workload": f"XTC decode, {F} frames x {N} atoms, bytes in HBM",
+                          "compressed_bytes": comp, "decoded_bytes": out_bytes,
+                          "decoded_GBps": out_bytes / (ms * 1e-3) / 1e9},
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+        if not args.no_cpu_baseline:
+            res["cpu_baseline"] = cpu_baseline_xtc(f, xyz, fn)
+        return res
+    finally:
+        os.unlink(fn)
+
+
+def cpu_baseline_xtc(f, xyz_dev, fn, sample_frames=20_000):
+    """The reference's own C reader (oracle/_ref/libref_xtc.so, compiled
+    from xdrfile.c / xdrfile_xtc.c) on the first frames, one host thread,
+    reading the sample from a file as the reference does; its output must
+    equal the GPU decode bit for bit."""
+    import ctypes
+    import tempfile
+
+    import numpy as np
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_xtc.so"))
+    n = min(len(f), sample_frames)
+    end = int(f._offsets[n]) if n < len(f) else f._host.size
+    fd, sfn = tempfile.mkstemp(suffix=".xtc")
+    with os.fdopen(fd, "wb") as fh:
+        fh.write(f._host[:end].tobytes())
+    N = f.n_atoms
+    bufs = [np.zeros((n + 1) * k, np.float32) for k in (3 * N, 1, 1, 9, 1)]
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    t0 = time.perf_counter()
+    got = lib.refxtc_read(sfn.encode(), N, n + 1, P(bufs[0]), P(bufs[1]), P(bufs[2]),
+                          P(bufs[3]), P(bufs[4]))
+    dt = time.perf_counter() - t0
+    os.unlink(sfn)
+    same = bool(np.array_equal(bufs[0][: 3 * N * got].reshape(got, N, 3),
+                               xyz_dev[:got].cpu().numpy()))
+    return {"value": got / dt, "unit": "frames/s", "cores": 1, "kind": "reference",
+            "sample": f"first {got} frames, read from a file by the reference's read_xtc loop",
+            "gpu_equals_reference": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +603,9 @@ def main():
     # reaches steady clocks before the timed region
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="one-vs-all", choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving"])
+    ap.add_argument("--workload", default="one-vs-all",
+                    choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving",
+                             "xtc-decode"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
@@ -515,7 +620,7 @@ def main():
                     help="lanes per frame of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
     per_workload = {"one-vs-all": (200, 100), "all-pairs": (5, 3), "batched": (50, 20),
-                    "hbm-ceiling": (20, 10), "serving": (0, 0)}
+                    "hbm-ceiling": (20, 10), "serving": (0, 0), "xtc-decode": (20, 5)}
     d_steps, d_warm = per_workload[args.workload]
     args.steps = d_steps if args.steps is None else args.steps
     args.warmup = d_warm if args.warmup is None else args.warmup
@@ -543,6 +648,9 @@ def main():
         return
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.workload == "xtc-decode":
+        print(json.dumps(bench_xtc_decode(args)), flush=True)
         return
     if args.workload == "one-vs-all":
         args.frames = args.frames or 100_000
