@@ -254,6 +254,33 @@ int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t
                   int32_t n_sel, int32_t mode, float* out, int64_t out_ld, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Consumers of the RMSD matrix (clustering), fused or on the condensed output
+ *
+ * hr_rmsd_cross_argmin replaces the reference's leader assignment loop
+ * `leader_labels[np.argmin(md.rmsd(leaders, frame, 0))]` over every frame
+ * (examples/two-pass-clustering.ipynb, cell 6): for each frame i of set A,
+ * the nearest frame of set B, index[i] = argmin_j RMSD(A_i, B_j) (the lowest
+ * j on ties, as np.argmin) and value[i] = that RMSD, with the minimum taken
+ * inside the cross kernel (no n_a x n_b matrix).  The workspace holds 8 n_a
+ * bytes (hr_rmsd_cross_argmin_workspace_bytes); no allocation.
+ *
+ * hr_centroid_scores replaces examples/centroids.ipynb, cells 3-4:
+ *   scores[i] = sum_j exp(-beta d_ij / std(d)),
+ * d the full symmetric n x n RMSD matrix given as its condensed upper
+ * triangle (hr_rmsd_all_pairs' output), d_ii = 0, std over all n^2 entries
+ * (numpy's population std); FP64 throughout; the centroid is the argmax.
+ * stats (optional, 3 doubles) receives sum(d_ij, i<j), sum(d_ij^2, i<j) and
+ * the std.  Workspace: hr_centroid_workspace_bytes(n).
+ * ------------------------------------------------------------------------- */
+size_t hr_rmsd_cross_argmin_workspace_bytes(int64_t n_a);
+int hr_rmsd_cross_argmin(const void* block_a, int64_t n_a, const void* block_b, int64_t n_b,
+                         int32_t n_sel, int32_t mode, int32_t* index, float* value,
+                         void* workspace, void* stream);
+size_t hr_centroid_workspace_bytes(int64_t n_frames);
+int hr_centroid_scores(const float* condensed, int64_t n_frames, double beta, double* scores,
+                       double* stats, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Batched one-vs-all over many independent systems of uneven size (no
  * counterpart in the reference, whose users loop md.rmsd per trajectory).
  * A plan captures, once, a hipGraph of three kernels (ticket reset, reference

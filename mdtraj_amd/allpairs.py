@@ -96,3 +96,58 @@ def rmsd_cross(a, b, out=None):
     call("hr_rmsd_cross", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
          a.n_sel, MODES[a.mode], out.data_ptr(), out.stride(0), dev.stream())
     return out
+
+
+def nearest(a, b):
+    """For every frame of prepared set ``a``, the nearest frame of set ``b``:
+    (index int32 tensor, RMSD float32 tensor), the minimum taken inside the
+    cross kernel (``hr_rmsd_cross_argmin``; ties go to the lowest index, as
+    ``np.argmin``).  The leader-assignment step of
+    examples/two-pass-clustering.ipynb (cell 6) with ``b`` the leaders."""
+    if a.mode != b.mode or a.n_sel != b.n_sel:
+        raise ValueError("sets must share the mode and the number of selected atoms")
+    if b.n_frames == 0 and a.n_frames:
+        raise ValueError("no frames to take the nearest of")
+    lib = load()
+    t = dev.torch()
+    index = t.empty(a.n_frames, dtype=t.int32, device=a.block.device)
+    value = t.empty(a.n_frames, dtype=t.float32, device=a.block.device)
+    ws = dev.workspace(lib.hr_rmsd_cross_argmin_workspace_bytes(a.n_frames))
+    call("hr_rmsd_cross_argmin", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
+         a.n_sel, MODES[a.mode], index.data_ptr(), value.data_ptr(), ws.data_ptr(), dev.stream())
+    return index, value
+
+
+def assign_to_leaders(frames, leaders, atom_indices=None, mode="f16exact"):
+    """Index of the nearest leader for every frame, and that RMSD (numpy):
+    ``argmin(md.rmsd(leaders, frame, 0))`` for all frames in one pass."""
+    idx, val = nearest(SplitSet(frames, atom_indices, mode), SplitSet(leaders, atom_indices, mode))
+    return dev.to_host(idx), dev.to_host(val)
+
+
+def centroid_scores(condensed, n_frames, beta=1.0):
+    """``sum_j exp(-beta d_ij / d.std())`` for every frame i of the full
+    symmetric matrix whose condensed upper triangle is ``condensed``
+    (examples/centroids.ipynb, cells 3-4); float64 numpy array."""
+    lib = load()
+    t = dev.torch()
+    n = int(n_frames)
+    if n < 0:
+        raise ValueError("n_frames must be >= 0")
+    D = dev.to_device_f32(condensed).reshape(-1)
+    if D.numel() != n * (n - 1) // 2:
+        raise ValueError(f"condensed length {D.numel()} does not match {n} frames")
+    scores = t.empty(max(n, 1), dtype=t.float64, device=D.device)
+    ws = dev.workspace(lib.hr_centroid_workspace_bytes(n))
+    call("hr_centroid_scores", D.data_ptr() if D.numel() else None, n, float(beta),
+         scores.data_ptr(), None, ws.data_ptr(), dev.stream())
+    return dev.to_host(scores[:n])
+
+
+def centroid(traj, atom_indices=None, beta=1.0, mode="f16exact"):
+    """Index of the frame with the largest summed similarity to all others
+    (examples/centroids.ipynb): all-pairs RMSD, then centroid_scores."""
+    xyz = traj.xyz if hasattr(traj, "xyz") else traj
+    n = int(xyz.shape[0])
+    d = rmsd_matrix(dev.to_device_f32(xyz), atom_indices=atom_indices, mode=mode)
+    return int(np.argmax(centroid_scores(d, n, beta)))

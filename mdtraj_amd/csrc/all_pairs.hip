@@ -288,6 +288,9 @@ struct GramArgs {
     long long tiles_b;   // rectangle: tile columns
     float* out;
     long long out_ld;    // rectangle: row stride of out
+    // rectangle, nearest-column mode: instead of writing out, every row i
+    // keeps min over its columns of (float bits of the RMSD) << 32 | j
+    unsigned long long* row_best;
 };
 
 constexpr int kMaxLoads = 7;  // loads per wave per stage (waves 0-1: 7, the rest: 6)
@@ -516,12 +519,39 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half
                     gb[h] = 3.5;
                 }
             }
-            if (!ok[0] && !ok[1]) continue;
-            double msd0, msd1;
-            qcp_msd2(M2[0], ga[0], gb[0], M2[1], ga[1], gb[1], n, msd0, msd1);
-            // one float rounding, then the float root
-            if (ok[0]) g.out[o2[0]] = sqrtf((float)msd0);
-            if (ok[1]) g.out[o2[1]] = sqrtf((float)msd1);
+            const bool any = ok[0] || ok[1];
+            if (!g.row_best && !any) continue;
+            double msd[2] = {0.0, 0.0};
+            if (any) qcp_msd2(M2[0], ga[0], gb[0], M2[1], ga[1], gb[1], n, msd[0], msd[1]);
+            if (!g.row_best) {
+                // one float rounding, then the float root
+                if (ok[0]) g.out[o2[0]] = sqrtf((float)msd[0]);
+                if (ok[1]) g.out[o2[1]] = sqrtf((float)msd[1]);
+                continue;
+            }
+            // nearest column of each row: the 16 lanes of a lane group share
+            // the row (i) and hold 16 consecutive columns (j); an RMSD is >= 0,
+            // so its float bits order like its value, and the column in the
+            // low word makes ties go to the lowest column
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int p = threadIdx.x + kMBlock * (k + h);
+                const int pw = p >> 8, v = (p >> 6) & 3, ln = p & 63;
+                const long long i = i0 + 32 * (pw >> 2) + 16 * rb + 4 * (ln >> 4) + v;
+                const long long j = j0 + 16 * (pw & 3) + (ln & 15);
+                unsigned long long key =
+                    ok[h] ? ((unsigned long long)__float_as_uint(sqrtf((float)msd[h])) << 32) |
+                                (unsigned long long)(unsigned)j
+                          : ~0ull;
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1) {
+                    const unsigned hi = __shfl_xor((unsigned)(key >> 32), m, kWave);
+                    const unsigned lo = __shfl_xor((unsigned)key, m, kWave);
+                    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+                    key = o < key ? o : key;
+                }
+                if ((ln & 15) == 0 && key != ~0ull) atomicMin(g.row_best + i, key);
+            }
         }
         __syncthreads();
     }
@@ -652,6 +682,60 @@ int cross_split(hipStream_t s, const void* a, long long n_a, const void* b, long
         return HR_ERR_UNSUPPORTED;
     }
     return launch_gram<Half, kSplit, true>(s, g);
+}
+
+// Nearest column per row: keys start at all ones (a fill kernel, so the
+// sequence can be captured in a graph), the rectangle kernel min-reduces
+// them, and this kernel splits them into the column index and the RMSD.
+__global__ __launch_bounds__(256) void fill_keys_kernel(unsigned long long* __restrict__ keys,
+                                                        long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) keys[i] = ~0ull;
+}
+
+__global__ __launch_bounds__(256) void unpack_keys_kernel(const unsigned long long* __restrict__ keys,
+                                                          long long n, int* __restrict__ index,
+                                                          float* __restrict__ value) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = keys[i];
+    index[i] = (int)(unsigned)k;
+    value[i] = __uint_as_float((unsigned)(k >> 32));
+}
+
+template <typename Half, Split kSplit>
+int cross_argmin(hipStream_t s, const void* a, long long n_a, const void* b, long long n_b,
+                 int n_sel, int K, int* index, float* value, unsigned long long* keys) {
+    SplitBlock ba = carve_split(static_cast<char*>(const_cast<void*>(a)), n_a, K);
+    SplitBlock bb = carve_split(static_cast<char*>(const_cast<void*>(b)), n_b, K);
+    GramArgs<Half> g{};
+    g.split_a = static_cast<const Half*>(ba.split);
+    g.split_b = static_cast<const Half*>(bb.split);
+    g.scale_a = ba.scale;
+    g.scale_b = bb.scale;
+    g.traces_a = ba.traces;
+    g.traces_b = bb.traces;
+    g.n_a = n_a;
+    g.n_b = n_b;
+    g.n_sel = n_sel;
+    g.k_pad = K;
+    g.tiles_b = (n_b + kMTile - 1) / kMTile;
+    g.n_blocks = (n_a + kMTile - 1) / kMTile * g.tiles_b;
+    g.out = nullptr;
+    g.out_ld = 0;
+    g.row_best = keys;
+    if (g.n_blocks >= (1LL << 31)) {
+        set_error("too many tiles (%lld)", g.n_blocks);
+        return HR_ERR_UNSUPPORTED;
+    }
+    const unsigned blocks = (unsigned)((n_a + 255) / 256);
+    hipLaunchKernelGGL(fill_keys_kernel, dim3(blocks), dim3(256), 0, s, keys, n_a);
+    int rc = launch_status("fill_keys_kernel");
+    if (rc) return rc;
+    if ((rc = launch_gram<Half, kSplit, true>(s, g))) return rc;
+    hipLaunchKernelGGL(unpack_keys_kernel, dim3(blocks), dim3(256), 0, s, keys, n_a, index,
+                       value);
+    return launch_status("unpack_keys_kernel");
 }
 
 bool split_mode(int mode) {
@@ -789,6 +873,41 @@ int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t
         default:
             return cross_split<__bf16, Split::ExactHi>(s, block_a, n_a, block_b, n_b, n_sel, K,
                                                        out, out_ld);
+    }
+}
+
+size_t hr_rmsd_cross_argmin_workspace_bytes(int64_t n_a) {
+    return n_a > 0 ? (size_t)n_a * sizeof(unsigned long long) : 256;
+}
+
+int hr_rmsd_cross_argmin(const void* block_a, int64_t n_a, const void* block_b, int64_t n_b,
+                         int32_t n_sel, int32_t mode, int32_t* index, float* value,
+                         void* workspace, void* stream) {
+    clear_error();
+    HR_REQUIRE(n_a >= 0 && n_b >= 0 && n_sel > 0, "invalid sizes n_a=%lld n_b=%lld n_sel=%d",
+               (long long)n_a, (long long)n_b, n_sel);
+    HR_REQUIRE(split_mode(mode), "mode %d is not a split mode", mode);
+    HR_REQUIRE(n_b > 0 || n_a == 0, "no columns to take the nearest of");
+    if (n_a == 0) return HR_OK;
+    HR_REQUIRE(n_b < (1LL << 31), "too many columns (%lld)", (long long)n_b);
+    HR_REQUIRE(block_a && block_b && index && value && workspace, "NULL pointer argument");
+    if (!current_device()) return HR_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const int K = k_pad_for(n_sel);
+    auto* keys = static_cast<unsigned long long*>(workspace);
+    switch (mode) {
+        case HR_PAIRS_F16X3:
+            return cross_argmin<_Float16, Split::ThreeProduct>(s, block_a, n_a, block_b, n_b,
+                                                               n_sel, K, index, value, keys);
+        case HR_PAIRS_BF16X3:
+            return cross_argmin<__bf16, Split::ThreeProduct>(s, block_a, n_a, block_b, n_b, n_sel,
+                                                             K, index, value, keys);
+        case HR_PAIRS_F16_EXACT:
+            return cross_argmin<_Float16, Split::ExactHi>(s, block_a, n_a, block_b, n_b, n_sel, K,
+                                                          index, value, keys);
+        default:
+            return cross_argmin<__bf16, Split::ExactHi>(s, block_a, n_a, block_b, n_b, n_sel, K,
+                                                        index, value, keys);
     }
 }
 

@@ -329,3 +329,25 @@ def brute_force_rmsd(a, b, n_grid=24, refine=200, seed=0):
         if step < 1e-10:
             break
     return np.sqrt(max(best, 0.0))
+
+
+def centroid_scores(D, beta=1.0):
+    """examples/centroids.ipynb cells 3-4 on a full square distance matrix:
+    sum_j exp(-beta D_ij / D.std()) in float64 (the notebook fills a float64
+    matrix with md.rmsd rows and takes numpy's population std)."""
+    D = np.asarray(D, np.float64)
+    return np.exp(-beta * D / D.std()).sum(axis=1)
+
+
+def nearest_leader(frames, leaders, atom_indices=None):
+    """examples/two-pass-clustering.ipynb cell 6: for every frame,
+    argmin over leaders of md.rmsd(leaders, frame, 0), here with this
+    oracle's rmsd; returns (index, rmsd) arrays."""
+    frames = np.asarray(frames)
+    idx = np.empty(len(frames), np.int64)
+    val = np.empty(len(frames), np.float64)
+    for f in range(len(frames)):
+        r = rmsd(leaders, frames[f:f + 1], 0, atom_indices)
+        idx[f] = int(np.argmin(r))
+        val[f] = r[idx[f]]
+    return idx, val
