@@ -320,9 +320,13 @@ def bench_batched(args):
     log("generating 512 uneven systems in HBM")
     xs = gen.uneven_systems_device(seed=3)
     torch.cuda.synchronize()
-    best, table = autotune.tune_batch(xs, reps=3)
-    log("autotuner: " + ", ".join(f"fpu={k[0]} kib={k[1]} waves={k[2]}: {v:.3f} ms"
-                                  for k, v in table.items()))
+    if args.batch_shape:
+        best = tuple(int(v) for v in args.batch_shape.split(","))
+        log(f"shape given: fpu={best[0]} kib={best[1]} waves={best[2]} (autotuner skipped)")
+    else:
+        best, table = autotune.tune_batch(xs, reps=3)
+        log("autotuner: " + ", ".join(f"fpu={k[0]} kib={k[1]} waves={k[2]}: {v:.3f} ms"
+                                      for k, v in table.items()))
     b = BatchRMSD(xs, frames_per_unit=best[0], unit_kib=best[1], waves=best[2])
     for _ in range(args.warmup):
         b.run()
@@ -596,6 +600,223 @@ def cpu_baseline_xtc(f, xyz_dev, fn, sample_frames=20_000):
             "gpu_equals_reference": same}
 
 
+def _time_events(fn, steps, warmup):
+    """Mean milliseconds of fn() over `steps` calls on torch's current stream
+    (HIP events), after `warmup` untimed calls."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def bench_nearest_leader(args):
+    """Leader assignment (examples/two-pass-clustering.ipynb, cell 6): the
+    nearest of L leaders for each of F frames, the minimum taken inside the
+    MFMA cross kernel (hr_rmsd_cross_argmin, f16exact).  Workload: the
+    random-walk trajectory, 100,000 frames x 500 atoms, leaders every 400th
+    frame (250).  Timed: the fused kernel with both sets prepared in HBM;
+    the unfused route (hr_rmsd_cross into an F x L matrix, then torch's
+    argmin) is timed beside it."""
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev, allpairs, gen
+
+    F, N, step_l = args.frames or 100_000, 500, 400
+    X = gen.walk_device(seed=3, n_frames=F, n_atoms=N)
+    a = allpairs.SplitSet(X, mode=args.mode)
+    b = allpairs.SplitSet(X[::step_l].contiguous(), mode=args.mode)
+    L = b.n_frames
+    lib = _lib.load()
+    mode = allpairs.MODES[args.mode]
+    idx = torch.empty(F, dtype=torch.int32, device=X.device)
+    val = torch.empty(F, dtype=torch.float32, device=X.device)
+    ws = dev.workspace(lib.hr_rmsd_cross_argmin_workspace_bytes(F))
+    full = torch.empty((F, L), dtype=torch.float32, device=X.device)
+
+    def fused():
+        _lib.call("hr_rmsd_cross_argmin", a.block.data_ptr(), F, b.block.data_ptr(), L, N, mode,
+                  idx.data_ptr(), val.data_ptr(), ws.data_ptr(), dev.stream())
+
+    def unfused():
+        _lib.call("hr_rmsd_cross", a.block.data_ptr(), F, b.block.data_ptr(), L, N, mode,
+                  full.data_ptr(), full.stride(0), dev.stream())
+        torch.argmin(full, dim=1)
+
+    ms = _time_events(fused, args.steps, args.warmup)
+    ms_unfused = _time_events(unfused, args.steps, args.warmup)
+    unfused()
+    assert torch.equal(idx.long(), torch.argmin(full, dim=1)), "fused argmin differs"
+    useful = 18.0 * N * F * L  # the nine-entry correlation of every pair
+    tflops = useful / (ms * 1e-3) / 1e12
+    log(f"nearest leader {F} x {L} leaders x {N} atoms: fused {ms:.3f} ms, unfused "
+        f"{ms_unfused:.3f} ms, {tflops:.1f} TFLOP/s useful")
+    return {"metric": "frames assigned to their nearest leader per second", "value": F / (ms * 1e-3),
+            "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": args.mode, "data": "synthetic (seeded random walk generated in HBM)",
+            "config": {"workload": f"nearest of {L} leaders for {F} frames x {N} atoms, mode {args.mode}",
+                       "unfused_ms": ms_unfused, "speedup_vs_unfused": ms_unfused / ms},
+            "roofline": {"bound": "mfma", "achieved": tflops, "peak": F16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": tflops / F16_PEAK_TFLOPS, "traffic": None}}
+
+
+def bench_centroid(args):
+    """Centroid scoring (examples/centroids.ipynb, cells 3-4) on the condensed
+    all-pairs matrix of the random-walk trajectory, 20,000 frames x 500
+    atoms (computed before the timed region).  Timed: hr_centroid_scores
+    (FP64 std, row and column sums of exp(-beta d / std))."""
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev, allpairs, gen
+
+    F, N = args.frames or 20_000, 500
+    X = gen.walk_device(seed=4, n_frames=F, n_atoms=N)
+    D = allpairs.rmsd_matrix(X)
+    lib = _lib.load()
+    scores = torch.empty(F, dtype=torch.float64, device=X.device)
+    ws = dev.workspace(lib.hr_centroid_workspace_bytes(F))
+
+    def run():
+        _lib.call("hr_centroid_scores", D.data_ptr(), F, 1.0, scores.data_ptr(), None,
+                  ws.data_ptr(), dev.stream())
+
+    ms = _time_events(run, args.steps, args.warmup)
+    n_pairs = F * (F - 1) // 2
+    algo = 3 * 4 * n_pairs  # the stats, row and column passes each read the matrix once
+    achieved = algo / (ms * 1e-3) / 1e9
+    log(f"centroid {F} frames: {ms:.3f} ms, {achieved:.0f} GB/s")
+    return {"metric": "all-pairs matrix entries scored per second (centroid)",
+            "value": n_pairs / (ms * 1e-3), "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded random walk generated in HBM)",
+            "config": {"workload": f"centroid scores over the {F}-frame condensed matrix",
+                       "centroid": int(torch.argmax(scores).item())},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+
+
+def bench_dcd_decode(args):
+    """DCD ingestion: the axis-major X/Y/Z records of every frame of a file
+    resident in HBM written as float32 (F, N, 3).  Input: a CHARMM-format
+    file (unit cell in every frame) of the random-walk trajectory of the
+    one-vs-all workload, 100,000 frames x 2,000 atoms by default, in the
+    layout the reference's writer produces (gen.dcd_image).  Timed:
+    hr_dcd_decode alone (HIP events), bytes already in HBM."""
+    import numpy as np
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev, gen
+
+    F, N = args.frames or 100_000, 2000
+    src = gen.walk_device(seed=11, n_frames=F, n_atoms=N)
+    img = gen.dcd_image(src.cpu().numpy(), lengths=(60.0, 61.0, 62.0))
+    hdr = _lib.DcdHeader()
+    import ctypes
+
+    _lib.call("hr_dcd_scan", img.ctypes.data_as(ctypes.c_void_p), img.size, ctypes.byref(hdr),
+              None, 0)
+    d = dev.device()
+    data = torch.from_numpy(img).to(d)
+    sample = img[: hdr.first_frame + min(F, 10_000) * hdr.frame_bytes]
+    del img
+    frames = torch.arange(F, dtype=torch.int64, device=d)
+    xyz = torch.empty((F, N, 3), dtype=torch.float32, device=d)
+    cell = torch.empty((F, 6), dtype=torch.float32, device=d)
+    status = torch.empty(F, dtype=torch.int32, device=d)
+
+    def run():
+        _lib.call("hr_dcd_decode", data.data_ptr(), data.numel(), ctypes.byref(hdr),
+                  frames.data_ptr(), F, None, N, None, 1.0, xyz.data_ptr(), cell.data_ptr(),
+                  status.data_ptr(), dev.stream())
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(args.steps):
+        run()
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / args.steps
+    assert int(status.max().item()) == 0, "decode reported a malformed frame"
+    assert torch.equal(xyz, src), "decoded coordinates differ from the ones written"
+    # per frame: 12 N coordinate bytes, 24 marker bytes and the 56-byte cell
+    # record read; 12 N coordinate bytes, the 24-byte cell and a status word written
+    per_frame = (12 * N + 24 + 56) + (12 * N + 24 + 4)
+    achieved = F * per_frame / (ms * 1e-3) / 1e9
+    log(f"dcd decode {F} frames x {N} atoms: {ms:.3f} ms, {F / ms * 1e3:.3e} frames/s, "
+        f"{achieved:.0f} GB/s")
+    res = {"metric": "DCD frames decoded/s (bit-identical to the reference reader)",
+           "value": F / (ms * 1e-3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic random walk written in the reference writer's layout",
+           "config": {"workload": f"DCD decode, {F} frames x {N} atoms with a unit cell, bytes in HBM",
+                      "file_bytes": int(data.numel()), "bytes_per_frame": per_frame},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_dcd(sample, xyz, N)
+    return res
+
+
+def cpu_baseline_dcd(sample, xyz_dev, N):
+    """The reference's own C reader (oracle/_ref/libref_dcd.so, compiled
+    from dcdplugin.c) reading the first frames from a file, one host thread,
+    as DCDTrajectoryFile.read drives it; its output must equal the GPU
+    decode bit for bit."""
+    import ctypes
+    import tempfile
+
+    import numpy as np
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_dcd.so"))
+    fd, fn = tempfile.mkstemp(suffix=".dcd")
+    with os.fdopen(fd, "wb") as fh:
+        fh.write(sample.tobytes())
+    na, ns = ctypes.c_int(), ctypes.c_int()
+    # the reference library reports on stdout; keep the JSON line alone there
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        lib.refdcd_info(fn.encode(), ctypes.byref(na), ctypes.byref(ns))
+        n = ns.value
+        out = np.zeros((n, N, 3), np.float32)
+        cell = np.zeros((n, 6), np.float32)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            got = lib.refdcd_read(fn.encode(), 0, 1, n, P(out), P(cell))
+            reps += 1
+            if time.perf_counter() - t0 > 10.0:
+                break
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+        os.unlink(fn)
+    same = bool(np.array_equal(out[:got], xyz_dev[:got].cpu().numpy()))
+    return {"value": got / dt, "unit": "frames/s", "cores": 1, "kind": "reference",
+            "sample": f"first {got} frames, read from a file (page cache) by the reference's "
+                      f"read_next_timestep loop, {reps} passes",
+            "gpu_equals_reference": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -605,11 +826,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="one-vs-all",
                     choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving",
-                             "xtc-decode"])
+                             "xtc-decode", "dcd-decode", "nearest-leader", "centroid"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-shape", default=None,
+                    help="batched workload: frames_per_unit,unit_kib,waves_per_simd instead of "
+                         "autotuning (e.g. 64,192,3), so that a profile holds only that shape")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
                     help="register budget of the per-frame kernel (0 = library default)")
@@ -620,7 +844,8 @@ def main():
                     help="lanes per frame of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
     per_workload = {"one-vs-all": (200, 100), "all-pairs": (5, 3), "batched": (50, 20),
-                    "hbm-ceiling": (20, 10), "serving": (0, 0), "xtc-decode": (20, 5)}
+                    "hbm-ceiling": (20, 10), "serving": (0, 0), "xtc-decode": (20, 5),
+                    "dcd-decode": (50, 10), "nearest-leader": (20, 5), "centroid": (20, 5)}
     d_steps, d_warm = per_workload[args.workload]
     args.steps = d_steps if args.steps is None else args.steps
     args.warmup = d_warm if args.warmup is None else args.warmup
@@ -651,6 +876,15 @@ def main():
         return
     if args.workload == "xtc-decode":
         print(json.dumps(bench_xtc_decode(args)), flush=True)
+        return
+    if args.workload == "dcd-decode":
+        print(json.dumps(bench_dcd_decode(args)), flush=True)
+        return
+    if args.workload == "nearest-leader":
+        print(json.dumps(bench_nearest_leader(args)), flush=True)
+        return
+    if args.workload == "centroid":
+        print(json.dumps(bench_centroid(args)), flush=True)
         return
     if args.workload == "one-vs-all":
         args.frames = args.frames or 100_000

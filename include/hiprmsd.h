@@ -350,6 +350,63 @@ int hr_xtc_decode(const void* data, int64_t n_bytes, const int64_t* offsets, int
                   float* prec, int32_t* status, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * DCD ingestion: CHARMM / NAMD / X-PLOR trajectory files decoded on the device
+ *
+ * Replaces the reference's frame loop over read_next_timestep
+ * (formats/dcd/dcd.pyx:463-551 -> formats/dcd/src/dcdplugin.c:935-1013,
+ * read_dcdstep :481-623): the file's bytes are copied to HBM once and every
+ * selected atom of every requested frame is written as float32 (F, n_sel, 3).
+ * ------------------------------------------------------------------------- */
+/* hr_dcd_header.flags, the reference's feature flags (dcdplugin.h) */
+#define HR_DCD_CHARMM 0x01      /* CHARMM-style header (a version number is set) */
+#define HR_DCD_FOUR_DIMS 0x02   /* a fourth coordinate record follows Z (skipped) */
+#define HR_DCD_EXTRA_BLOCK 0x04 /* every frame starts with a unit-cell record */
+/* Per-frame status words of hr_dcd_decode. */
+#define HR_DCD_OK 0
+#define HR_DCD_BAD_RECORD 1 /* a record length marker is not what the header implies */
+#define HR_DCD_TRUNCATED 2  /* the frame lies outside the bytes, or is not in the file */
+
+typedef struct hr_dcd_header {
+    int32_t n_atoms;           /* N */
+    int32_t n_fixed;           /* NAMNF: atoms stored only in frame 0 */
+    int64_t n_frames;          /* frames in the file, from its size (dcdplugin.c:870-905) */
+    int64_t first_frame;       /* byte offset of frame 0 */
+    int64_t first_frame_bytes; /* bytes of frame 0 (every atom) */
+    int64_t frame_bytes;       /* bytes of every later frame (the free atoms) */
+    int32_t flags;             /* HR_DCD_* */
+    int32_t reverse_endian;    /* 1 when the file's byte order is not the host's */
+    int32_t n_set;             /* NSET as the header states it */
+    int32_t istart;            /* ISTART */
+    int32_t nsavc;             /* NSAVC */
+    int32_t pad;
+    double delta;              /* DELTA (a float in CHARMM headers, a double in X-PLOR ones) */
+} hr_dcd_header;
+
+/* Header of DCD bytes in HOST memory (read_dcdheader, dcdplugin.c:98-374):
+ * fills *header, and the 1-based free-atom indices (n_atoms - n_fixed of
+ * them; free_indexes may be NULL with max_free 0).  n_bytes is the whole
+ * file's size: the frame count follows from it, as in the reference.  Files
+ * that are not DCD, or hold no frame, fail with HR_ERR_INVALID; CHARMM -i8
+ * files (64-bit record markers) with HR_ERR_UNSUPPORTED. */
+int hr_dcd_scan(const void* data, int64_t n_bytes, hr_dcd_header* header, int32_t* free_indexes,
+                int32_t max_free);
+/* Decode frames[0..n_frames) (device int64 frame numbers, any subset of the
+ * file's frames) of DCD bytes in DEVICE memory (4-byte aligned):
+ * xyz (n_frames, n_sel, 3) = the atoms atom_indices[0..n_sel) (device int32;
+ * NULL for all atoms, n_sel == n_atoms) times `scale` in float32 (1 keeps
+ * the file's angstroms; 0.1f gives nanometres exactly as the reference's
+ * in-place conversion); cell (n_frames, 6) float32, may be NULL: the raw
+ * record (A, cos gamma, B, cos beta, cos alpha, C) cast to float, or
+ * (0, 90, 0, 90, 90, 0) without one; status (n_frames HR_DCD_* words).
+ * With fixed atoms, free_slot (device int32[n_atoms]) gives each atom's
+ * position in the free list, or -1 for a fixed atom.  Atom indices outside
+ * [0, n_atoms) give NaN coordinates. */
+int hr_dcd_decode(const void* data, int64_t n_bytes, const hr_dcd_header* header,
+                  const int64_t* frames, int64_t n_frames, const int32_t* atom_indices,
+                  int32_t n_sel, const int32_t* free_slot, float scale, float* xyz, float* cell,
+                  int32_t* status, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device
  * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
  * achieved rate is the HBM read ceiling the RMSD kernels are compared with.

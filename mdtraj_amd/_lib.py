@@ -70,6 +70,9 @@ _SIGNATURES = {
     "hr_centroid_workspace_bytes": (_sz, [_i64]),
     "hr_centroid_scores": (ctypes.c_int, [_vp, _i64, ctypes.c_double, _vp, _vp, _vp, _vp]),
     "hr_xtc_decode": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "hr_dcd_scan": (ctypes.c_int, [_vp, _i64, _vp, _vp, _i32]),
+    "hr_dcd_decode": (ctypes.c_int, [_vp, _i64, _vp, _vp, _i64, _vp, _i32, _vp, ctypes.c_float,
+                                     _vp, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()
@@ -81,6 +84,16 @@ class SystemDesc(ctypes.Structure):
     _fields_ = [("xyz", ctypes.c_void_p), ("n_frames", ctypes.c_int64),
                 ("n_atoms", ctypes.c_int32), ("ref_frame", ctypes.c_int32),
                 ("out", ctypes.c_void_p)]
+
+
+class DcdHeader(ctypes.Structure):
+    """hr_dcd_header of include/hiprmsd.h."""
+    _fields_ = [("n_atoms", ctypes.c_int32), ("n_fixed", ctypes.c_int32),
+                ("n_frames", ctypes.c_int64), ("first_frame", ctypes.c_int64),
+                ("first_frame_bytes", ctypes.c_int64), ("frame_bytes", ctypes.c_int64),
+                ("flags", ctypes.c_int32), ("reverse_endian", ctypes.c_int32),
+                ("n_set", ctypes.c_int32), ("istart", ctypes.c_int32),
+                ("nsavc", ctypes.c_int32), ("pad", ctypes.c_int32), ("delta", ctypes.c_double)]
 
 
 class BatchShape(ctypes.Structure):

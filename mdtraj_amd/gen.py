@@ -186,3 +186,36 @@ def uneven_sizes(seed=3, n_systems=512, atoms=(50, 5000), frames=(10, 2000)):
     na = log_uniform_int(rng, atoms[0], atoms[1], n_systems)
     nf = log_uniform_int(rng, frames[0], frames[1], n_systems)
     return nf, na
+
+
+def dcd_image(xyz, lengths=None):
+    """Bytes of a CHARMM-format DCD file holding float32 coordinates xyz
+    (F, N, 3), in angstroms, with an orthorhombic unit cell of ``lengths``
+    (3 values) in every frame when given: the layout the reference's writer
+    produces (formats/dcd/src/dcdplugin.c:677-790: an 84-byte control
+    record, two title lines, the atom count, then per frame the cell record
+    and the X, Y and Z records), built with array operations so that large
+    synthetic files are cheap.  Returns a uint8 numpy array."""
+    import struct
+
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    F, N = xyz.shape[:2]
+    ctl = bytearray(80)
+    struct.pack_into("<iii", ctl, 0, F, 0, 1)  # NSET, ISTART, NSAVC
+    struct.pack_into("<f", ctl, 36, 1.0)  # DELTA
+    struct.pack_into("<i", ctl, 40, 0 if lengths is None else 1)  # unit cell in every frame
+    struct.pack_into("<i", ctl, 76, 24)  # CHARMM version
+    titles = b"synthetic trajectory".ljust(80) + b"mdtraj_amd.gen.dcd_image".ljust(80)
+    head = (struct.pack("<i", 84) + b"CORD" + bytes(ctl) + struct.pack("<iii", 84, 164, 2)
+            + titles + struct.pack("<iiii", 164, 4, N, 4))
+    cw = 0 if lengths is None else 14
+    rec = np.empty((F, cw + 3 * (N + 2)), np.uint32)
+    if cw:
+        a, b, c = (float(v) for v in lengths)
+        rec[:, 0] = rec[:, 13] = 48
+        rec[:, 1:13] = np.array([a, 0.0, b, 0.0, 0.0, c], np.float64).view(np.uint32)
+    for k in range(3):
+        o = cw + k * (N + 2)
+        rec[:, o] = rec[:, o + N + 1] = 4 * N
+        rec[:, o + 1:o + 1 + N] = xyz[:, :, k].view(np.uint32)
+    return np.concatenate([np.frombuffer(head, np.uint8), rec.reshape(-1).view(np.uint8)])

@@ -1,0 +1,130 @@
+"""DCD reader, host side: the header parse (hr_dcd_scan) against the
+oracle's on every fixture, host validation of malformed headers, the
+reader's position API, and a loud failure (no CPU decode) without a GPU."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from mdtraj_amd import _lib
+from mdtraj_amd.dcd import DCDTrajectoryFile, cell_lengths_angles
+from oracle import dcd as odcd
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+G = np.load(os.path.join(HERE, "dcd_golden.npz"))
+CASES = sorted({k.split("/")[0] for k in G.files})
+
+
+def path(name):
+    return os.path.join(HERE, f"dcd_{name}.dcd")
+
+
+def scan(buf):
+    lib = _lib.load()
+    a = np.frombuffer(buf, np.uint8)
+    h = _lib.DcdHeader()
+    free = np.zeros(4096, np.int32)
+    rc = lib.hr_dcd_scan(a.ctypes.data_as(ctypes.c_void_p), a.size, ctypes.byref(h),
+                         free.ctypes.data_as(ctypes.c_void_p), free.size)
+    return rc, h, free[: max(0, h.n_atoms - h.n_fixed)], lib.hr_last_error().decode()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_scan_matches_oracle_header(name):
+    buf = open(path(name), "rb").read()
+    rc, h, free, _ = scan(buf)
+    o = odcd.parse_header(buf)
+    assert rc == 0
+    assert (h.n_atoms, h.n_fixed, h.n_frames) == (o["natoms"], o["namnf"], o["n_frames"])
+    assert (h.first_frame, h.first_frame_bytes, h.frame_bytes) == (o["first"], o["first_bytes"],
+                                                                   o["frame_bytes"])
+    assert h.flags == o["flags"] and h.reverse_endian == (o["endian"] == ">")
+    assert (h.n_set, h.istart, h.nsavc, h.delta) == (o["nset"], o["istart"], o["nsavc"], o["delta"])
+    if o["free"] is not None:
+        assert np.array_equal(free, o["free"])
+    assert h.n_frames == int(G[f"{name}/nsets"])
+
+
+def test_scan_rejects_bad_bytes():
+    buf = open(path("charmm_cell"), "rb").read()
+    rc, _, _, msg = scan(buf[:50])
+    assert rc == _lib.HR_ERR_INVALID and "truncated" in msg
+    rc, _, _, msg = scan(buf[:4] + b"XXXX" + buf[8:])
+    assert rc == _lib.HR_ERR_INVALID and "unrecognized" in msg
+    o = odcd.parse_header(buf)
+    rc, _, _, msg = scan(buf[: o["first"] + o["first_bytes"] - 4])
+    assert rc == _lib.HR_ERR_INVALID and "no frames" in msg
+    i8 = (84).to_bytes(8, "little") + buf[4:]  # a 64-bit first marker
+    rc, _, _, msg = scan(i8)
+    assert rc == _lib.HR_ERR_UNSUPPORTED and "-i8" in msg
+    bad_free = bytearray(open(path("fixed_degrees"), "rb").read())
+    of = odcd.parse_header(bytes(bad_free))["first"] - 4 - 4 * 20  # first free index
+    bad_free[of:of + 4] = (31).to_bytes(4, "little")  # atom 31 of 30
+    rc, _, _, msg = scan(bytes(bad_free))
+    assert rc == _lib.HR_ERR_INVALID and "out of range" in msg
+
+
+def test_cell_conversion_matches_reference_values():
+    for name in CASES:
+        raw = G[f"{name}/cell"]  # the reference's (A, B, C, alpha, beta, gamma)
+        if not len(raw):
+            continue
+        # rebuild the raw record from the oracle's read of the file and convert it here
+        d = open(path(name), "rb").read()
+        h = odcd.parse_header(d)
+        cells = []
+        for k in range(len(raw)):
+            r = odcd.read_frame(d, h, k, odcd.read_frame(d, h, 0)[0])
+            cells.append(r[1])
+        L, A = cell_lengths_angles(np.array(cells))
+        assert np.array_equal(L, raw[:, :3]) and np.array_equal(A, raw[:, 3:])
+
+
+def test_reader_position_api():
+    with DCDTrajectoryFile(path("charmm_cell")) as f:
+        assert len(f) == 12 and f.n_atoms == 23 and f.tell() == 0
+        assert f.distance_unit == "angstroms"
+        f.seek(5)
+        f.seek(3, whence=1)
+        assert f.tell() == 8
+        f.seek(-2, whence=1)
+        assert f.tell() == 6
+        f.seek(-20, whence=1)
+        assert f.tell() == 0
+        f.seek(40)  # past the end stops at the end
+        assert f.tell() == 12
+        with pytest.raises(NotImplementedError):
+            f.seek(-1, whence=2)
+        with pytest.raises(IOError):
+            f.seek(-1)
+    with pytest.raises(RuntimeError):
+        len(f)
+    with pytest.raises(ValueError):
+        DCDTrajectoryFile(path("xplor"), mode="w")
+    with pytest.raises(IOError):
+        DCDTrajectoryFile(path("missing"))
+
+
+def test_no_cpu_decode():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with DCDTrajectoryFile(path("xplor")) as f:
+        with pytest.raises(RuntimeError):
+            f.read()
+
+
+def test_synthetic_image_is_a_dcd_file():
+    from mdtraj_amd import gen
+
+    x = gen.walk(seed=5, n_frames=13, n_atoms=31) * np.float32(10)
+    img = gen.dcd_image(x, lengths=(40.0, 41.5, 42.0))
+    xyz, L, A, _ = odcd.read(img.tobytes())
+    assert np.array_equal(xyz, x)
+    assert np.all(L == np.float32([40.0, 41.5, 42.0])) and np.all(A == 90.0)
+    rc, h, _, _ = scan(img.tobytes())
+    assert rc == 0 and h.n_frames == 13 and h.flags == 5
+    xyz, L, _, _ = odcd.read(gen.dcd_image(x).tobytes())
+    assert np.array_equal(xyz, x) and L is None
