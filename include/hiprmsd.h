@@ -267,7 +267,8 @@ int hr_rmsd_cross(const void* block_a, int64_t n_a, const void* block_b, int64_t
  * hr_centroid_scores replaces examples/centroids.ipynb, cells 3-4:
  *   scores[i] = sum_j exp(-beta d_ij / std(d)),
  * d the full symmetric n x n RMSD matrix given as its condensed upper
- * triangle (hr_rmsd_all_pairs' output), d_ii = 0, std over all n^2 entries
+ * triangle (hr_rmsd_all_pairs' output, device, 16-byte aligned), d_ii = 0,
+ * std over all n^2 entries
  * (numpy's population std); FP64 throughout; the centroid is the argmax.
  * stats (optional, 3 doubles) receives sum(d_ij, i<j), sum(d_ij^2, i<j) and
  * the std.  Workspace: hr_centroid_workspace_bytes(n).

@@ -135,6 +135,8 @@ def centroid_scores(condensed, n_frames, beta=1.0):
     if n < 0:
         raise ValueError("n_frames must be >= 0")
     D = dev.to_device_f32(condensed).reshape(-1)
+    if D.data_ptr() % 16:  # the kernel reads 16-byte groups
+        D = D.clone()
     if D.numel() != n * (n - 1) // 2:
         raise ValueError(f"condensed length {D.numel()} does not match {n} frames")
     scores = t.empty(max(n, 1), dtype=t.float64, device=D.device)

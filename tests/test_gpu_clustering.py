@@ -85,3 +85,16 @@ def test_centroid_two_clusters_and_large_n(gpu):
     D = square(cond.astype(np.float64), len(y))
     want = np.exp(-D / D.std()).sum(axis=1)
     assert np.allclose(got[sub], want[sub], rtol=1e-9)
+
+
+def test_centroid_scores_of_a_misaligned_view(gpu):
+    import torch
+
+    x = gen.walk(seed=37, n_frames=300, n_atoms=30)
+    cond = allpairs.rmsd_matrix(x)
+    buf = torch.zeros(cond.size + 1, dtype=torch.float32, device="cuda")
+    buf[1:] = torch.from_numpy(cond).cuda()
+    view = buf[1:]  # 4 bytes past a 16-byte boundary: the wrapper copies it
+    assert view.data_ptr() % 16 == 4
+    got = allpairs.centroid_scores(view, 300)
+    assert np.array_equal(got, allpairs.centroid_scores(cond, 300))
