@@ -408,6 +408,30 @@ int hr_dcd_decode(const void* data, int64_t n_bytes, const hr_dcd_header* header
                   int32_t* status, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * LP-RMSD: the RMSD minimised over rotations and over relabellings of atoms
+ * inside permute groups
+ *
+ * Replaces the frame loop of md.lprmsd (rmsd/_lprmsd.pyx:137-228) and the
+ * assignment of euclidean_permutation (rmsd/src/euclidean_permutation.cpp:
+ * 17-71, Munkres): per frame, the selection is centred; when some selected
+ * atoms are in no group, they alone are fitted and the selection rotated;
+ * each group's labels are assigned by an exact minimum-cost assignment of
+ * squared distances; the RMSD is the QCP RMSD of the relabelled selection.
+ * atom_indices (n_sel, sorted, unique), dis (the n_dis positions in the
+ * selection of atoms in no group) and the groups (positions in the
+ * selection, group g at group_atoms[group_offsets[g] .. group_offsets[g+1]),
+ * max_group the largest) are device arrays; ref_frame is the reference
+ * frame (n_atoms, 3), device.  With superpose, every frame of xyz is centred
+ * (all atoms) and rotated in place, as the reference.  A selection and group
+ * too large for one wave's LDS (64 KiB) fail with HR_ERR_UNSUPPORTED.
+ * ------------------------------------------------------------------------- */
+size_t hr_lprmsd_workspace_bytes(int32_t n_sel, int32_t n_dis);
+int hr_lprmsd(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ref_frame,
+              const int32_t* atom_indices, int32_t n_sel, const int32_t* dis, int32_t n_dis,
+              const int32_t* group_atoms, const int32_t* group_offsets, int32_t n_groups,
+              int32_t max_group, int32_t superpose, float* out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device
  * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
  * achieved rate is the HBM read ceiling the RMSD kernels are compared with.

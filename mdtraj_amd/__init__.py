@@ -5,6 +5,8 @@ Drop-in surface (names and semantics of ``mdtraj``):
          parallel=True, precentered=False, superpose=True)
     rmsf(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
          parallel=True, precentered=False, mode='atom')
+    lprmsd(target, reference, frame=0, atom_indices=None, permute_groups=None,
+           parallel=True, superpose=False)
     Trajectory(xyz).superpose(...), .center_coordinates(), ._rmsd_traces
     _rmsd: the ``mdtraj._rmsd`` module functions
     capi(): paths of the C header and shared library (as ``md.capi()``)
@@ -16,6 +18,7 @@ from __future__ import annotations
 import os
 
 from . import _rmsd
+from ._lprmsd import lprmsd
 from ._rmsd import rmsd, rmsf
 from .trajectory import Trajectory
 
@@ -36,4 +39,4 @@ def rmsd_matrix(*args, **kwargs):
     return _rm(*args, **kwargs)
 
 
-__all__ = ["rmsd", "rmsf", "Trajectory", "capi", "rmsd_matrix", "_rmsd"]
+__all__ = ["rmsd", "rmsf", "lprmsd", "Trajectory", "capi", "rmsd_matrix", "_rmsd"]

@@ -15,6 +15,10 @@
 #include "rotation.h"
 #include "theobald_rmsd.h"
 
+#include <vector>
+
+#include "euclidean_permutation.hpp"
+
 extern "C" {
 
 /* rmsd/_rmsd.pyx:197-224 with separate (gathered, contiguous) target and
@@ -83,3 +87,70 @@ float ref_msd_axis_major(int n_atoms, int rowstride, const float* aT, const floa
 }
 
 }  // extern "C"
+
+extern "C" {
+
+/* rmsd/_lprmsd.pyx:137-228: LP-RMSD of every frame of target (n_frames,
+ * n_total, 3) to ref_frame (n_total, 3) over atom_indices[0..n_sel) (sorted,
+ * unique), with permute groups given as indices into the selection
+ * (group_rel[group_off[g] .. group_off[g + 1]]).  With superpose, every frame
+ * of target is centred (all atoms) and rotated in place, as the reference.
+ * The arithmetic is the reference's: inplace_center_and_trace_atom_major,
+ * msd_atom_major, rot_atom_major, euclidean_permutation (Munkres), sgemm33. */
+void ref_lprmsd(float* target, int n_frames, int n_total, const float* ref_frame,
+                const int* atom_indices, int n_sel, const int* group_rel, const int* group_off,
+                int n_groups, int superpose, float* out) {
+    std::vector<std::vector<int> > groups(n_groups);
+    std::vector<int> in_group(n_sel, 0);
+    for (int g = 0; g < n_groups; ++g)
+        for (int k = group_off[g]; k < group_off[g + 1]; ++k) {
+            groups[g].push_back(group_rel[k]);
+            in_group[group_rel[k]] = 1;
+        }
+    std::vector<int> dis;
+    for (int i = 0; i < n_sel; ++i)
+        if (!in_group[i]) dis.push_back(i);
+    const int n_dis = (int)dis.size();
+    std::vector<float> ref(3 * n_sel), tf(3 * n_sel), tf2(3 * n_sel), td(3 * (n_dis + 1)),
+        refd(3 * (n_dis + 1));
+    for (int i = 0; i < n_sel; ++i)
+        for (int d = 0; d < 3; ++d) ref[3 * i + d] = ref_frame[3 * atom_indices[i] + d];
+    float ref_g = 0.0f, ref_g_dis = 0.0f;
+    inplace_center_and_trace_atom_major(ref.data(), &ref_g, 1, n_sel);
+    if (n_dis > 0) {
+        for (int k = 0; k < n_dis; ++k)
+            for (int d = 0; d < 3; ++d) refd[3 * k + d] = ref_frame[3 * atom_indices[dis[k]] + d];
+        inplace_center_and_trace_atom_major(refd.data(), &ref_g_dis, 1, n_dis);
+    }
+    for (int i = 0; i < n_frames; ++i) {
+        float* fr = target + (size_t)i * n_total * 3;
+        for (int a = 0; a < n_sel; ++a)
+            for (int d = 0; d < 3; ++d) tf[3 * a + d] = fr[3 * atom_indices[a] + d];
+        float tg = 0.0f;
+        inplace_center_and_trace_atom_major(tf.data(), &tg, 1, n_sel);
+        float rot1[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, rot2[9], rot3[9];
+        if (n_dis > 0) {
+            for (int k = 0; k < n_dis; ++k)
+                for (int d = 0; d < 3; ++d) td[3 * k + d] = tf[3 * dis[k] + d];
+            float tgd = 0.0f;
+            inplace_center_and_trace_atom_major(td.data(), &tgd, 1, n_dis);
+            msd_atom_major(n_dis, n_dis, td.data(), refd.data(), tgd, ref_g_dis, 1, rot1);
+            rot_atom_major(n_sel, tf.data(), rot1);
+        }
+        std::vector<int> mapping = euclidean_permutation(ref.data(), tf.data(), n_sel, 3, groups);
+        for (int a = 0; a < n_sel; ++a)
+            for (int d = 0; d < 3; ++d) tf2[3 * a + d] = tf[3 * mapping[a] + d];
+        float msd;
+        if (superpose) {
+            msd = msd_atom_major(n_sel, n_sel, tf2.data(), ref.data(), tg, ref_g, 1, rot2);
+            inplace_center_and_trace_atom_major(fr, NULL, 1, n_total);
+            sgemm33(rot1, rot2, rot3);
+            rot_atom_major(n_total, fr, rot3);
+        } else {
+            msd = msd_atom_major(n_sel, n_sel, ref.data(), tf2.data(), tg, ref_g, 0, NULL);
+        }
+        out[i] = sqrtf(msd);
+    }
+}
+
+}  // extern "C" (LP-RMSD)

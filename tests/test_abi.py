@@ -70,6 +70,8 @@ def test_invalid_arguments_are_rejected_on_the_host():
          "not a split mode"),
         ("hr_dcd_decode", (None, 0, None, None, 0, None, 0, None, 1.0, None, None, None, None),
          "NULL pointer"),
+        ("hr_lprmsd", (None, 1, 10, None, None, 12, None, 0, None, None, 0, 0, 0, None, None, None),
+         "invalid sizes"),
     ]
     for name, args, text in cases:
         rc, msg = _status(lib, name, *args)
@@ -87,3 +89,11 @@ def test_batch_shape_is_validated():
         rc, msg = _status(lib, "hr_batch_plan_create", ctypes.addressof(desc), 1,
                           ctypes.addressof(s), ctypes.byref(plan))
         assert rc == 1 and text in msg, (shape, rc, msg)
+
+
+def test_lprmsd_lds_limit_is_a_host_error():
+    """A group too large for one wave's LDS is refused before any launch."""
+    lib = _lib.load()
+    rc, msg = _status(lib, "hr_lprmsd", None, 1, 3000, None, None, 3000, None, 0, None, None, 1,
+                      3000, 0, None, None, None)
+    assert rc == _lib.HR_ERR_UNSUPPORTED and "LDS" in msg
