@@ -16,7 +16,8 @@
 //       superpose, the frame (every atom, centred) is rotated by R1 R2.
 // One wave per frame: the columns of a search are spread over the lanes,
 // the search's minimum is a wave reduction (ties to the lowest column), and
-// the group's potentials and the search state sit in LDS.
+// the group's reference coordinates, potentials and search state sit in
+// LDS (sized to keep eight waves per CU for groups of a few hundred atoms).
 #include "common.h"
 
 namespace hr {
@@ -32,8 +33,18 @@ __host__ __device__ inline size_t lp_lds_bytes(int n_sel, int kmax) {
     return 3 * 8 * k1      // u, v, minv
            + 2 * 4 * k1    // p, way
            + 4 * (size_t)kmax  // the group's atoms
+           + 12 * (size_t)kmax  // the group's reference coordinates
            + 12 * (size_t)n_sel  // target selection, x, y, z
            + k1;           // used
+}
+
+// The workgroup is one wave, whose LDS accesses complete in program order:
+// ordering them needs only a wavefront fence that keeps the compiler from
+// moving them, not a workgroup barrier (which would also wait for memory).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Index arrays live on the device, where the host cannot check them: every
@@ -108,18 +119,35 @@ struct LpArgs {
     float* out;
 };
 
-// Minimum (value, column) of the wave, ties to the lower column; a NaN never
-// beats a number but a valid column always beats none (INT_MAX).
-__device__ __forceinline__ void wave_argmin(double& best, int& col) {
-#pragma unroll
-    for (int m = kWave / 2; m >= 1; m >>= 1) {
-        const double ob = __shfl_xor(best, m, kWave);
-        const int oc = __shfl_xor(col, m, kWave);
-        if (ob < best || (!(ob > best) && oc < col)) {
-            best = ob;
-            col = oc;
-        }
+// (value, column) pairs ordered by value, then by the lower column; a NaN
+// never beats a number, and any valid column beats none (INT_MAX).  The
+// order is total on the pairs that occur, so any reduction tree gives the
+// same minimum.
+__device__ __forceinline__ void take_min(double& best, int& col, double ob, int oc) {
+    if (ob < best || (!(ob > best) && oc < col)) {
+        best = ob;
+        col = oc;
     }
+}
+
+template <int kCtrl>
+__device__ __forceinline__ void argmin_dpp(double& best, int& col) {
+    const double ob = dpp_move<kCtrl>(best);
+    const int oc = __builtin_amdgcn_mov_dpp(col, kCtrl, 0xF, 0xF, false);
+    take_min(best, col, ob, oc);
+}
+
+// Minimum of the wave, in every lane: within rows of 16 lanes by DPP moves
+// (xor 1, xor 2, half mirror, mirror: a few cycles each), then across rows
+// by two lane exchanges.
+__device__ __forceinline__ void wave_argmin(double& best, int& col) {
+    argmin_dpp<0xB1>(best, col);
+    argmin_dpp<0x4E>(best, col);
+    argmin_dpp<0x141>(best, col);
+    argmin_dpp<0x140>(best, col);
+#pragma unroll
+    for (int m = 16; m <= 32; m <<= 1) take_min(best, col, __shfl_xor(best, m, kWave),
+                                                __shfl_xor(col, m, kWave));
 }
 
 __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
@@ -131,7 +159,10 @@ __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
     int* p = reinterpret_cast<int*>(minv + k1);
     int* way = p + k1;
     int* gat = way + k1;
-    float* tx = reinterpret_cast<float*>(gat + A.kmax);
+    float* grx = reinterpret_cast<float*>(gat + A.kmax);  // the group's reference coordinates
+    float* gry = grx + A.kmax;
+    float* grz = gry + A.kmax;
+    float* tx = grz + A.kmax;
     float* ty = tx + A.n_sel;
     float* tz = ty + A.n_sel;
     unsigned char* used = reinterpret_cast<unsigned char*>(tz + A.n_sel);
@@ -239,37 +270,42 @@ __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
             if (lane == 0) A.out[f] = __builtin_nanf("");
             return;
         }
-        for (int j = lane; j < k; j += kWave) gat[j] = clamp_index(A.group_atoms[g0 + j], n);
+        for (int j = lane; j < k; j += kWave) {
+            const int a = clamp_index(A.group_atoms[g0 + j], n);
+            gat[j] = a;
+            grx[j] = rx[a];
+            gry[j] = ry[a];
+            grz[j] = rz[a];
+        }
         for (int j = lane; j <= k; j += kWave) {
             u[j] = 0.0;
             v[j] = 0.0;
             p[j] = 0;
             way[j] = 0;
         }
-        __syncthreads();
+        wave_sync();
         for (int i = 1; i <= k; ++i) {
             for (int j = lane; j <= k; j += kWave) {
                 minv[j] = __builtin_huge_val();
                 used[j] = 0;
             }
             if (lane == 0) p[0] = i;
-            __syncthreads();
+            wave_sync();
             int j0 = 0;
             // each pass marks one more column used; a free column (p = 0) is
             // reached within i passes, so the loop ends for any costs
             for (int pass = 0; pass <= k; ++pass) {
                 if (lane == 0) used[j0] = 1;
-                __syncthreads();
+                wave_sync();
                 const int i0 = p[j0];
-                const int ra = gat[i0 - 1];
-                const float qx = rx[ra], qy = ry[ra], qz = rz[ra];
+                const float qx = grx[i0 - 1], qy = gry[i0 - 1], qz = grz[i0 - 1];
                 const double ui = u[i0];
                 double best = __builtin_huge_val();
                 int col = INT_MAX;
                 for (int j = 1 + lane; j <= k; j += kWave) {
                     if (used[j]) continue;
-                    const int ta = gat[j - 1];
                     // euclidean_permutation.cpp: squares of float differences, summed in double
+                    const int ta = gat[j - 1];
                     const float dx = qx - tx[ta], dy = qy - ty[ta], dz = qz - tz[ta];
                     const double c = (double)__fmul_rn(dx, dx) + (double)__fmul_rn(dy, dy) +
                                      (double)__fmul_rn(dz, dz);
@@ -287,7 +323,7 @@ __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
                 }
                 wave_argmin(best, col);
                 const double delta = best;
-                __syncthreads();  // every lane has read u[i0] before it changes
+                wave_sync();  // every lane has read u[i0] before it changes
                 for (int j = lane; j <= k; j += kWave) {
                     if (used[j]) {
                         u[p[j]] += delta;
@@ -296,7 +332,7 @@ __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
                         minv[j] -= delta;
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 j0 = col;
                 if (p[j0] == 0) break;
             }
@@ -308,19 +344,19 @@ __global__ __launch_bounds__(kLpBlock) void lp_frame_kernel(LpArgs A) {
                     jj = j1;
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
         // column j (target atom gat[j - 1]) takes row p[j] (reference atom gat[p[j] - 1])
         for (int j = 1 + lane; j <= k; j += kWave) {
-            const int ta = gat[j - 1], ra = gat[p[j] - 1];
+            const int ta = gat[j - 1], ra = p[j] - 1;
             const double t[3] = {tx[ta], ty[ta], tz[ta]};
-            const double r[3] = {rx[ra], ry[ra], rz[ra]};
+            const double r[3] = {grx[ra], gry[ra], grz[ra]};
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) M2[3 * a + b] = fma(t[a], r[b], M2[3 * a + b]);
         }
-        __syncthreads();  // the next group overwrites gat and p
+        wave_sync();  // the next group overwrites the group's arrays
     }
 
     // (4) QCP on the relabelled selection
