@@ -54,24 +54,75 @@ __device__ __forceinline__ uint32_t be32_word(const uint32_t* w) {
 // sizeofint, and sizeofints of a product).
 __device__ __forceinline__ int bit_length64(uint64_t v) { return v ? 64 - __clzll(v) : 0; }
 
+// floor(v / d) and v mod d for v < 2^50 from a double reciprocal: the
+// estimate is within one of the quotient (relative error below 2^-51 on a
+// quotient below 2^50), and one integer step corrects it.
+__device__ __forceinline__ uint64_t div_by_recip(uint64_t v, uint32_t d, double inv_d,
+                                                 uint32_t& rem) {
+    uint64_t q = (uint64_t)((double)v * inv_d);
+    long long r = (long long)v - (long long)(q * d);
+    if (r < 0) {
+        q -= 1;
+        r += d;
+    } else if (r >= (long long)d) {
+        q += 1;
+        r -= d;
+    }
+    rem = (uint32_t)r;
+    return q;
+}
+
 // MSB-first reader over one frame's bit stream (whole big-endian words;
 // the stream is word-aligned and padded to a word in the file).
+// The words come in blocks of kBlockWords registers loaded together: a lane
+// reads its frame's words in order, and with few waves per SIMD (one lane
+// per frame) nothing hides a load's latency, so each block costs one
+// latency where word-by-word loads cost one per word.
+constexpr int kBlockWords = 16;
+
 struct BitReader {
     const uint32_t* w;
     int n_words;
-    int next;       // next word to load
+    int consumed;   // words moved into buf
+    int left;       // words of blk not yet moved into buf (blk[0] is next)
+    uint32_t blk[kBlockWords];
     uint64_t buf;   // the low `have` bits are unread
     int have;
     bool bad;
 
+    __device__ __forceinline__ void start(const uint32_t* words, int count) {
+        w = words;
+        n_words = count;
+        consumed = 0;
+        left = 0;
+        buf = 0;
+        have = 0;
+        bad = false;
+    }
+
+    __device__ __forceinline__ void refill() {
+        const int avail = n_words - consumed;  // > 0: word `consumed` exists
+#pragma unroll
+        for (int i = 0; i < kBlockWords; ++i) {
+            const uint32_t x = w[consumed + (i < avail ? i : 0)];
+            blk[i] = i < avail ? x : 0u;
+        }
+        left = avail < kBlockWords ? avail : kBlockWords;
+    }
+
     __device__ __forceinline__ uint32_t take(int n) {  // 0 <= n <= 32
         if (n == 0) return 0;
         if (have < n) {
-            if (next >= n_words) {
+            if (consumed >= n_words) {
                 bad = true;
                 return 0;
             }
-            buf = (buf << 32) | be32_word(w + next++);
+            if (left == 0) refill();
+            buf = (buf << 32) | __builtin_bswap32(blk[0]);
+#pragma unroll
+            for (int i = 0; i < kBlockWords - 1; ++i) blk[i] = blk[i + 1];
+            --left;
+            ++consumed;
             have += 32;
         }
         have -= n;
@@ -81,22 +132,47 @@ struct BitReader {
     // Three digits (x, y, z) of one mixed-radix number of nbits bits whose
     // 8-bit chunks come least significant first: z = V mod sz, y = (V / sz)
     // mod sy, x = V / (sz sy) (low 32 bits).
+    // inv_sz, inv_sy: 1 / sz and 1 / sy in double when the divisors are the
+    // frame's large-atom sizes (0 otherwise).
     __device__ __forceinline__ void ints3(int nbits, uint32_t sx, uint32_t sy, uint32_t sz,
-                                          int32_t (&out)[3]) {
+                                          double inv_sz, double inv_sy, int32_t (&out)[3]) {
         (void)sx;
         if (nbits <= 64) {
-            uint64_t v = 0;
-            int shift = 0;
-            while (nbits > 8) {
-                v |= (uint64_t)take(8) << shift;
-                shift += 8;
-                nbits -= 8;
+            // the chunks are a = (nbits - 1) / 8 whole bytes, then the last
+            // b = nbits - 8 a bits: read the field at once (most significant
+            // bit first) and reverse the byte order of its whole bytes
+            uint64_t w;
+            if (nbits > 32) {
+                const uint64_t hi = take(nbits - 32);
+                w = (hi << 32) | take(32);
+            } else {
+                w = take(nbits);
             }
-            if (nbits > 0) v |= (uint64_t)take(nbits) << shift;
-            const uint64_t q = v / sz;
-            out[2] = (int32_t)(v - q * sz);
-            const uint64_t q2 = q / sy;
-            out[1] = (int32_t)(q - q2 * sy);
+            const int a = (nbits - 1) >> 3, b = nbits - 8 * a;
+            uint64_t v = w & ((1ull << b) - 1);
+            if (a) v = (v << (8 * a)) | (__builtin_bswap64(w >> b) >> (64 - 8 * a));
+            if (nbits <= 32) {  // the common case for small atoms: 32-bit division
+                const uint32_t v32 = (uint32_t)v;
+                const uint32_t q = v32 / sz;
+                out[2] = (int32_t)(v32 - q * sz);
+                const uint32_t q2 = q / sy;
+                out[1] = (int32_t)(q - q2 * sy);
+                out[0] = (int32_t)q2;
+                return;
+            }
+            uint64_t q, q2;
+            uint32_t r, r2;
+            if (inv_sz > 0.0 && v < (1ull << 50)) {  // the large atoms of a frame
+                q = div_by_recip(v, sz, inv_sz, r);
+                q2 = div_by_recip(q, sy, inv_sy, r2);
+            } else {
+                q = v / sz;
+                r = (uint32_t)(v - q * sz);
+                q2 = q / sy;
+                r2 = (uint32_t)(q - q2 * sy);
+            }
+            out[2] = (int32_t)r;
+            out[1] = (int32_t)r2;
             out[0] = (int32_t)(uint32_t)q2;
             return;
         }
@@ -219,7 +295,9 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
     int smallnum = kMagicInts[smallidx] / 2;
     uint32_t sizesmall = (uint32_t)kMagicInts[smallidx];
     const float inv = (float)(1.0 / (double)precision);
-    BitReader br{pw + 23, (int)((nbytes + 3) / 4), 0, 0ull, 0, false};
+    const double inv_size2 = 1.0 / (double)sizeint[2], inv_size1 = 1.0 / (double)sizeint[1];
+    BitReader br;
+    br.start(pw + 23, (int)((nbytes + 3) / 4));
     int n_out = 0;
     int run = 0;  // a clear flag bit keeps the previous run length
     while (n_out < n_atoms) {
@@ -227,7 +305,7 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
         if (large) {
             for (int k = 0; k < 3; ++k) big[k] = (int32_t)br.take(bitsizeint[k]);
         } else {
-            br.ints3(bitsize, sizeint[0], sizeint[1], sizeint[2], big);
+            br.ints3(bitsize, sizeint[0], sizeint[1], sizeint[2], inv_size2, inv_size1, big);
         }
         for (int k = 0; k < 3; ++k) big[k] = (int32_t)((uint32_t)big[k] + (uint32_t)minint[k]);
         int is_smaller = 0;
@@ -253,7 +331,7 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
             int32_t prev[3] = {big[0], big[1], big[2]};
             for (int k = 0; k < run; k += 3) {
                 int32_t d[3];
-                br.ints3(smallidx, sizesmall, sizesmall, sizesmall, d);
+                br.ints3(smallidx, sizesmall, sizesmall, sizesmall, 0.0, 0.0, d);
                 for (int c = 0; c < 3; ++c)
                     prev[c] = (int32_t)((uint32_t)d[c] + (uint32_t)prev[c] - (uint32_t)smallnum);
                 put_atom(out + 3 * n_out++, prev, inv);
