@@ -787,6 +787,45 @@ def cpu_baseline_lprmsd(X, ref, out_dev, n_sol, N, sample=20):
             "max_abs_diff_vs_gpu": float(np.abs(gpu - o).max())}
 
 
+def bench_superpose(args):
+    """Trajectory.superpose of the one-vs-all workload: 100,000 frames x
+    2,000 atoms in HBM, fitted on the 1,500-atom subset to frame 0 and every
+    atom moved in place (hr_superpose, translate=1: the fit kernel, then the
+    rotation kernel).  Each timed call moves the frames again (a superposed
+    trajectory superposed once more), so the work per call is the same."""
+    import ctypes
+
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev, gen
+
+    F, N, NSEL = args.frames or 100_000, 2000, 1500
+    X = gen.walk_device(seed=1, n_frames=F, n_atoms=N)
+    idx = torch.from_numpy(gen.subset(2, N, NSEL).astype("int32")).to(X.device)
+    R = X[0].clone()
+    lib = _lib.load()
+    ws = dev.workspace(lib.hr_superpose_workspace_bytes(F, NSEL))
+
+    def run():
+        _lib.call("hr_superpose", X.data_ptr(), F, N, None, N, idx.data_ptr(), NSEL, R.data_ptr(),
+                  idx.data_ptr(), 1, None, None, ws.data_ptr(), dev.stream())
+
+    ms = _time_events(run, args.steps, args.warmup)
+    # fit pass: the frames' lines (12 B per atom, every line of the 75 % subset);
+    # rotation pass: every atom read and written
+    algo = 12.0 * F * N + 2 * 12.0 * F * N
+    achieved = algo / (ms * 1e-3) / 1e9
+    log(f"superpose {F} x {N} (fit on {NSEL}): {ms:.3f} ms per call, {achieved:.0f} GB/s")
+    return {"metric": "frames superposed per second (Trajectory.superpose)",
+            "value": F / (ms * 1e-3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded random walk generated in HBM)",
+            "config": {"workload": f"superpose {F} frames x {N} atoms on a {NSEL}-atom subset, in place"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+
+
 def bench_dcd_decode(args):
     """DCD ingestion: the axis-major X/Y/Z records of every frame of a file
     resident in HBM written as float32 (F, N, 3).  Input: a CHARMM-format
@@ -909,7 +948,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="one-vs-all",
                     choices=["one-vs-all", "all-pairs", "hbm-ceiling", "batched", "serving",
-                             "xtc-decode", "dcd-decode", "nearest-leader", "centroid", "lprmsd"])
+                             "xtc-decode", "dcd-decode", "nearest-leader", "centroid", "lprmsd", "superpose"])
     ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--mode", default="f16exact",
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
@@ -929,7 +968,7 @@ def main():
     per_workload = {"one-vs-all": (200, 100), "all-pairs": (5, 3), "batched": (50, 20),
                     "hbm-ceiling": (20, 10), "serving": (0, 0), "xtc-decode": (20, 5),
                     "dcd-decode": (50, 10), "nearest-leader": (20, 5), "centroid": (20, 5),
-                    "lprmsd": (5, 2)}
+                    "lprmsd": (5, 2), "superpose": (20, 5)}
     d_steps, d_warm = per_workload[args.workload]
     args.steps = d_steps if args.steps is None else args.steps
     args.warmup = d_warm if args.warmup is None else args.warmup
@@ -972,6 +1011,9 @@ def main():
         return
     if args.workload == "lprmsd":
         print(json.dumps(bench_lprmsd(args)), flush=True)
+        return
+    if args.workload == "superpose":
+        print(json.dumps(bench_superpose(args)), flush=True)
         return
     if args.workload == "one-vs-all":
         args.frames = args.frames or 100_000
