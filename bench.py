@@ -975,7 +975,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "path is a design (frames shard across GPUs with no exchange in the hot loop), "
-              "not part of this build.", file=sys.stderr)
+              "not implemented.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -33,7 +33,7 @@ extern "C" {
 #define HR_OK 0
 #define HR_ERR_INVALID 1     /* bad argument (size, pointer, index range) */
 #define HR_ERR_HIP 2         /* a HIP runtime call failed */
-#define HR_ERR_UNSUPPORTED 3 /* configuration this build does not handle */
+#define HR_ERR_UNSUPPORTED 3 /* a configuration the library does not handle */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int hr_version(void);

@@ -10,8 +10,13 @@ Drop-in surface (names and semantics of ``mdtraj``):
     Trajectory(xyz).superpose(...), .center_coordinates(), ._rmsd_traces
     _rmsd: the ``mdtraj._rmsd`` module functions
     capi(): paths of the C header and shared library (as ``md.capi()``)
+Readers (the reference's reading API, decoded on the GPU):
+    xtc.XTCTrajectoryFile, xtc.load_xtc; dcd.DCDTrajectoryFile, dcd.load_dcd
 Beyond the reference:
     rmsd_matrix(traj, ...)   all-pairs RMSD (FP32 or split-precision MFMA)
+    allpairs.assign_to_leaders / nearest, allpairs.centroid_scores / centroid
+                             consumers of the matrix (clustering notebooks)
+    batch.BatchRMSD, serve.RMSDServer   uneven batches and continuous batching
 """
 from __future__ import annotations
 
