@@ -15,8 +15,8 @@ prep, one cross-Gram pass over the pool, copy of the results to pinned host
 memory) on the next stream of a small ring as soon as a slot is free and the
 HBM budget admits its buffers.  Requests that arrive while earlier batches
 are on the GPU join later batches; batches on different streams overlap on
-the device.  A completion thread waits for each batch's event and hands every
-request its rows.  Each (query, pool frame) result depends only on those two
+the device.  A completion thread retires batches as their events complete,
+in whatever order that is, and hands every request its rows.  Each (query, pool frame) result depends only on those two
 frames (``hr_rmsd_cross``), so a request's answer is bit-identical whether
 it was scored alone or batched with others.
 """
@@ -218,6 +218,9 @@ class RMSDServer:
             self.done_cv.notify_all()
 
     def _completer(self):
+        # batches on different streams finish out of launch order: retire
+        # whichever has finished, so a small batch never waits behind a
+        # larger one launched before it
         try:
             while True:
                 with self.done_cv:
@@ -227,8 +230,16 @@ class RMSDServer:
                         if not self.running and not self._sched.is_alive():
                             return
                         continue
-                    ev, batch, host_out, keep, si, need, nq, nf = self.completions.popleft()
-                ev.synchronize()
+                    item = None
+                    for i, c in enumerate(self.completions):
+                        if c[0].query():
+                            item = c
+                            del self.completions[i]
+                            break
+                if item is None:
+                    time.sleep(2e-5)
+                    continue
+                ev, batch, host_out, keep, si, need, nq, nf = item
                 now = time.perf_counter()
                 res = host_out.numpy()
                 row = 0
