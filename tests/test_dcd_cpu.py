@@ -128,3 +128,32 @@ def test_synthetic_image_is_a_dcd_file():
     assert rc == 0 and h.n_frames == 13 and h.flags == 5
     xyz, L, _, _ = odcd.read(gen.dcd_image(x).tobytes())
     assert np.array_equal(xyz, x) and L is None
+
+
+def test_scan_and_oracle_agree_on_mutated_headers():
+    """Random byte changes and truncations of every fixture's header: the
+    C parser and the oracle accept and refuse the same inputs, with the same
+    fields when both accept, and neither reads past the bytes given."""
+    rng = np.random.default_rng(11)
+    fields = ("natoms", "namnf", "n_frames", "first", "first_bytes", "frame_bytes", "flags")
+    for name in CASES:
+        base = open(path(name), "rb").read()
+        for trial in range(60):
+            buf = bytearray(base)
+            if trial % 3 == 0:
+                buf = buf[: int(rng.integers(0, min(len(buf), 400)))]
+            else:
+                for _ in range(int(rng.integers(1, 4))):
+                    pos = int(rng.integers(0, min(len(buf), 300)))
+                    buf[pos] = int(rng.integers(0, 256))
+            data = bytes(buf)
+            rc, h, free, _ = scan(data)
+            try:
+                o = odcd.parse_header(data)
+            except odcd.DcdError:
+                o = None
+            assert (rc == 0) == (o is not None), (name, trial, rc)
+            if o is not None:
+                got = (h.n_atoms, h.n_fixed, h.n_frames, h.first_frame, h.first_frame_bytes,
+                       h.frame_bytes, h.flags)
+                assert got == tuple(o[k] for k in fields), (name, trial)
