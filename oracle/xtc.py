@@ -189,15 +189,23 @@ def frame_offsets(data):
     pos = 0
     n = len(data)
     while pos + 4 <= n:
+        if pos + 56 > n:
+            raise XtcError(f"truncated frame header at byte {pos}")
         (magic, natoms) = struct.unpack_from(">ii", data, pos)
         if magic != MAGIC:
             raise XtcError(f"bad magic {magic} at byte {pos}")
+        if natoms <= 0:
+            raise XtcError(f"frame at byte {pos} holds {natoms} atoms")
         offs.append(pos)
         c = pos + 16 + 36 + 4  # header, box, atom count
         if natoms <= 9:
             pos = c + 12 * natoms
         else:
+            if c + 36 > n:
+                raise XtcError(f"truncated frame at byte {pos}")
             (nbytes,) = struct.unpack_from(">i", data, c + 32)
+            if nbytes < 0:
+                raise XtcError(f"negative byte count in frame at byte {pos}")
             pos = c + 36 + (nbytes + 3) // 4 * 4
         if pos > n:
             raise XtcError("truncated last frame")
