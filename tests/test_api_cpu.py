@@ -81,3 +81,21 @@ def test_default_ref_indices_checked_against_reference():
     big, small = traj(N=12), traj(N=5)
     with pytest.raises(ValueError, match="ref_atom_indices must be valid positive indices"):
         md.rmsd(big, small, 0, atom_indices=[0, 8])
+
+
+def test_lprmsd_validation_on_the_host():
+    """md.lprmsd checks its arguments before any device work, with the
+    reference's messages (_lprmsd.pyx:232-271)."""
+    import mdtraj_amd as md
+
+    x = np.zeros((3, 6, 3), np.float32)
+    with pytest.raises(ValueError, match="only 3 frames"):
+        md.lprmsd(x, x, 7)
+    with pytest.raises(ValueError, match="same number of atoms"):
+        md.lprmsd(x, np.zeros((1, 5, 3), np.float32))
+    with pytest.raises(ValueError, match="valid positive indices"):
+        md.lprmsd(x, x, 0, atom_indices=[0, 9])
+    with pytest.raises(ValueError, match="subset of atom_indices"):
+        md.lprmsd(x, x, 0, atom_indices=[0, 1, 2], permute_groups=[[4]])
+    with pytest.raises(ValueError, match="mutually disjoint"):
+        md.lprmsd(x, x, 0, permute_groups=[[0, 1], [1, 2]])
