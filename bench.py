@@ -46,15 +46,19 @@ def cpu_threads():
     return int(n) if n and n.isdigit() else aff
 
 
-def traffic_from_profile(name):
+def traffic_from_profile(name, shape=None):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel of known size), committed under
-    profiles/; None when that measurement is absent."""
+    profiles/; None when that measurement is absent or was taken on another
+    shape than this run's."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        rec = json.load(f)
+    if shape is not None and "shape" in rec and list(rec["shape"]) != list(shape):
+        return None
+    return rec.get("hbm_bytes_per_launch")
 
 
 # --------------------------------------------------------------- one-vs-all
@@ -561,7 +565,8 @@ def bench_xtc_decode(args):
                           "compressed_bytes": comp, "decoded_bytes": out_bytes,
                           "decoded_GBps": out_bytes / (ms * 1e-3) / 1e9},
                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                            "traffic": traffic_from_profile("xtc_decode_traffic.json", [F, N])}}
         if not args.no_cpu_baseline:
             res["cpu_baseline"] = cpu_baseline_xtc(f, xyz, fn)
         return res
@@ -701,7 +706,8 @@ def bench_centroid(args):
             "config": {"workload": f"centroid scores over the {F}-frame condensed matrix",
                        "centroid": int(torch.argmax(scores).item())},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": traffic_from_profile("centroid_traffic.json", [F])}}
 
 
 def bench_lprmsd(args):
@@ -823,7 +829,8 @@ def bench_superpose(args):
             "data": "synthetic (seeded random walk generated in HBM)",
             "config": {"workload": f"superpose {F} frames x {N} atoms on a {NSEL}-atom subset, in place"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": traffic_from_profile("superpose_traffic.json", [F, N])}}
 
 
 def bench_dcd_decode(args):
@@ -886,7 +893,8 @@ def bench_dcd_decode(args):
            "config": {"workload": f"DCD decode, {F} frames x {N} atoms with a unit cell, bytes in HBM",
                       "file_bytes": int(data.numel()), "bytes_per_frame": per_frame},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": traffic_from_profile("dcd_decode_traffic.json", [F, N])}}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_dcd(sample, xyz, N)
     return res
