@@ -433,10 +433,12 @@ int hr_lprmsd(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ref_fr
 
 /* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device
- * memory with a plain float4 streaming kernel of `blocks` x 256 threads; the
+ * memory (16-byte aligned, a multiple of 16 bytes) with a float4 streaming
+ * kernel of `blocks` x 256 threads, with plain or nontemporal loads; the
  * achieved rate is the HBM read ceiling the RMSD kernels are compared with.
  * ------------------------------------------------------------------------- */
 int hr_hbm_read_probe(const void* data, size_t bytes, float* sink, int blocks, void* stream);
+int hr_hbm_read_probe_nt(const void* data, size_t bytes, float* sink, int blocks, void* stream);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "hr_batch_plan_units": (_i64, [_vp]),
     "hr_hbm_read_probe": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
+    "hr_hbm_read_probe_nt": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
     "hr_xtc_scan": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp]),
     "hr_rmsd_cross_argmin_workspace_bytes": (_sz, [_i64]),
     "hr_rmsd_cross_argmin": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp]),

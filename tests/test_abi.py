@@ -72,6 +72,12 @@ def test_invalid_arguments_are_rejected_on_the_host():
          "NULL pointer"),
         ("hr_lprmsd", (None, 1, 10, None, None, 12, None, 0, None, None, 0, 0, 0, None, None, None),
          "invalid sizes"),
+        ("hr_hbm_read_probe", (ctypes.addressof(ws), 24, ctypes.addressof(ws), 1, None),
+         "multiple of 16"),
+        ("hr_hbm_read_probe_nt", (ctypes.addressof(ws) + 4, 16, ctypes.addressof(ws), 1, None),
+         "16-byte aligned"),
+        ("hr_hbm_read_probe_nt", (ctypes.addressof(ws), 16, ctypes.addressof(ws), 0, None),
+         "invalid probe arguments"),
     ]
     for name, args, text in cases:
         rc, msg = _status(lib, name, *args)
