@@ -817,8 +817,9 @@ def bench_superpose(args):
     """Trajectory.superpose of the one-vs-all workload: 100,000 frames x
     2,000 atoms in HBM, fitted on the 1,500-atom subset to frame 0 and every
     atom moved in place (hr_superpose, translate=1: the fit kernel, then the
-    rotation kernel).  Each timed call moves the frames again (a superposed
-    trajectory superposed once more), so the work per call is the same."""
+    rotation kernel, per chunk of 1 GiB).  Each timed call moves the frames
+    again (a superposed trajectory superposed once more), so the work per
+    call is the same."""
     import ctypes
 
     import torch

@@ -67,12 +67,20 @@ int hr_device_cu_count(void);
  *                                  index and the rest from an atomic ticket;
  *                                  2 a static split of all systems' frames
  *                                  into one contiguous, cost-balanced range
- *                                  per wave (no tile shape to tune) */
+ *                                  per wave (no tile shape to tune)
+ *   HR_TUNE_SUPERPOSE_CHUNK_MIB    hr_superpose fits and rotates the frames
+ *                                  in chunks of about this many MiB of
+ *                                  coordinates (a fit pass, then a rotation
+ *                                  pass, per chunk); 0 the default (1024), a
+ *                                  value at least the trajectory's size one
+ *                                  fit pass and one rotation pass over all
+ *                                  frames.  Results do not depend on it. */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2
 #define HR_TUNE_BATCH_SCHEDULE 3
-#define HR_TUNE_COUNT 4
+#define HR_TUNE_SUPERPOSE_CHUNK_MIB 4
+#define HR_TUNE_COUNT 5
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 

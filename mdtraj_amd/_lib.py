@@ -21,6 +21,7 @@ HR_TUNE_FRAMES_WAVES_PER_SIMD = 0
 HR_TUNE_FRAMES_GRID = 1
 HR_TUNE_FRAMES_ROW_LANES = 2
 HR_TUNE_BATCH_SCHEDULE = 3
+HR_TUNE_SUPERPOSE_CHUNK_MIB = 4
 
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1

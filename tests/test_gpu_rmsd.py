@@ -268,6 +268,42 @@ def test_superpose_semantics(gpu):
     assert np.all(np.isfinite(t.xyz))
 
 
+def test_superpose_chunks_match_one_pass(gpu):
+    """hr_superpose fits and rotates in chunks of HR_TUNE_SUPERPOSE_CHUNK_MIB.
+    The boundaries change nothing: moved coordinates, rotations and RMSDs
+    equal a single pass bit for bit, a ragged last chunk included, and the
+    RMSDs match the FP64 oracle frame by frame."""
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev
+
+    F, N, frame = 500, 1000, 3  # 12,000 B per frame: 87 frames per MiB, 65 in the last chunk
+    w = gen.walk(seed=9, n_frames=F, n_atoms=N)
+    sel = np.arange(0, N, 3)
+    idx = torch.from_numpy(sel.astype(np.int32)).cuda()
+    x0 = torch.from_numpy(w).cuda()
+    ref = x0[frame].clone()
+    ws = dev.workspace(_lib.load().hr_superpose_workspace_bytes(F, len(sel)))
+    got = {}
+    try:
+        for mib in (4096, 1):
+            _lib.call("hr_set_tuning", _lib.HR_TUNE_SUPERPOSE_CHUNK_MIB, mib)
+            x = x0.clone()
+            rot = torch.empty(F, 9, device="cuda")
+            out = torch.empty(F, device="cuda")
+            _lib.call("hr_superpose", x.data_ptr(), F, N, None, N, idx.data_ptr(), len(sel),
+                      ref.data_ptr(), idx.data_ptr(), 1, rot.data_ptr(), out.data_ptr(),
+                      ws.data_ptr(), dev.stream())
+            torch.cuda.synchronize()
+            got[mib] = (x.cpu().numpy(), rot.cpu().numpy(), out.cpu().numpy())
+    finally:
+        _lib.call("hr_set_tuning", _lib.HR_TUNE_SUPERPOSE_CHUNK_MIB, 0)
+    for a, b in zip(got[4096], got[1]):
+        assert np.array_equal(a, b)
+    assert np.abs(got[1][2] - fp64.rmsd(w, w, frame, sel)).max() < TOL_FP64
+    assert np.abs(got[1][0] - fp64.superpose(w, w, frame, sel)).max() < 2e-5
+
+
 def test_superpose_in_place(gpu):
     """The reference transforms the trajectory's own array
     (core/trajectory.py:1128 aliases self.xyz): a host array held elsewhere
