@@ -153,6 +153,11 @@ __global__ __launch_bounds__(kCBlock) void centroid_tile_kernel(
     const long long j = (long long)bj * kCTile + threadIdx.x;
     const int n_rows = (int)(n - i0 < kCTile ? n - i0 : kCTile);
     double col = 0.0;
+    if (n < 2) {  // no entries (and maybe no matrix): one tile of zero partials
+        if (threadIdx.x < n_rows) row_part[(long long)bj * n + i0 + threadIdx.x] = 0.0;
+        if (j < n) col_part[(long long)bi * n + j] = 0.0;
+        return;
+    }
     for (int r0 = 0; r0 < n_rows; r0 += kRowBatch) {
         float x[kRowBatch];
         bool on[kRowBatch];
@@ -160,7 +165,10 @@ __global__ __launch_bounds__(kCBlock) void centroid_tile_kernel(
         for (int u = 0; u < kRowBatch; ++u) {
             const long long i = i0 + r0 + u;
             on[u] = r0 + u < n_rows && j > i && j < n;
-            x[u] = on[u] ? d[cidx(i, j, n)] : 0.0f;
+            // every lane loads, an entry of its own or entry 0, so that the
+            // batch's loads are in flight together rather than one
+            // divergent branch (and one wait) at a time
+            x[u] = d[on[u] ? cidx(i, j, n) : 0];
         }
         double a[kRowBatch];
 #pragma unroll

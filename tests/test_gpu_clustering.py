@@ -71,6 +71,9 @@ def test_centroid_scores(gpu):
     y = gen.walk(seed=29, n_frames=2, n_atoms=20)
     two = allpairs.centroid_scores(allpairs.rmsd_matrix(y), 2, beta=0.7)
     assert np.allclose(two, 1 + np.exp(-1.4), rtol=1e-12)
+    # one frame: no entries, the score is the diagonal's exp(0)
+    one = allpairs.centroid_scores(np.zeros(0, np.float32), 1)
+    assert np.array_equal(one, [1.0])
 
 
 def test_centroid_two_clusters_and_large_n(gpu):
