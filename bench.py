@@ -248,22 +248,23 @@ def bench_one_vs_all(args):
 
 
 # ---------------------------------------------------------------- all-pairs
-def bench_all_pairs(args):
+ALL_PAIRS_PRODUCTS = {"fp32": 1, "f16x3": 3, "bf16x3": 3, "f16exact": 4, "bf16exact": 4}
+
+
+def time_all_pairs(X, mode_name, warmup, steps):
+    """Per-launch milliseconds of hr_rmsd_all_pairs over X (F, N, 3) in one
+    mode (HIP events around each launch on the stream it runs on), plus the
+    condensed output of the last launch."""
     import numpy as np
     import torch
 
-    from mdtraj_amd import gen
-    from mdtraj_amd.allpairs import MODES
     from mdtraj_amd import _lib, _device as dev
+    from mdtraj_amd.allpairs import MODES
 
-    F, N = args.frames, 500
-    log(f"generating walk trajectory {F} x {N} in HBM")
-    X = gen.walk_device(seed=2, n_frames=F, n_atoms=N)
-    torch.cuda.synchronize()
     lib = _lib.load()
-    mode = MODES[args.mode]
-    n_pairs = F * (F - 1) // 2
-    out = dev.empty_f32(n_pairs)
+    F, N = int(X.shape[0]), int(X.shape[1])
+    mode = MODES[mode_name]
+    out = dev.empty_f32(F * (F - 1) // 2)
     nbytes = lib.hr_rmsd_all_pairs_workspace_bytes(F, N, mode)
     ws = dev.workspace(nbytes)
     stream = torch.cuda.current_stream()
@@ -272,27 +273,118 @@ def bench_all_pairs(args):
         _lib.call("hr_rmsd_all_pairs", X.data_ptr(), F, N, None, N, mode, out.data_ptr(),
                   ws.data_ptr(), nbytes, dev.stream())
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
     t0 = time.perf_counter()
-    s.record(stream)
-    for _ in range(args.steps):
+    for s, e in ev:
+        s.record(stream)
         step()
-    e.record(stream)
+        e.record(stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms = s.elapsed_time(e) / args.steps
+    per = np.array([s.elapsed_time(e) for s, e in ev])
+    del ws
+    return per, elapsed, out
+
+
+def all_pairs_figures(F, N, mode_name, ms):
+    """Useful and executed TFLOP/s of one all-pairs launch of `ms` ms."""
     useful = 2.0 * (3 * F) ** 2 * N / 2   # upper triangle of the 3F x 3F Gram
-    tflops = useful / (ms * 1e-3) / 1e12
-    peak = FP32_PEAK_TFLOPS if args.mode == "fp32" else F16_PEAK_TFLOPS
-    # MFMA products per useful product: 4 (exact-hi), 3 (three-product split)
-    products = {"fp32": 1, "f16x3": 3, "bf16x3": 3, "f16exact": 4, "bf16exact": 4}[args.mode]
     k_pad = (N + 63) // 64 * 64
-    executed = products * useful * k_pad / N
-    log(f"all-pairs {args.mode}: {ms:.3f} ms, useful {tflops:.1f} TFLOP/s, executed "
-        f"{executed / (ms * 1e-3) / 1e12:.1f} TFLOP/s")
+    executed = ALL_PAIRS_PRODUCTS[mode_name] * useful * k_pad / N
+    peak = FP32_PEAK_TFLOPS if mode_name == "fp32" else F16_PEAK_TFLOPS
+    return {"useful_tflops": useful / (ms * 1e-3) / 1e12,
+            "executed_tflops": executed / (ms * 1e-3) / 1e12,
+            "peak_tflops": peak,
+            "useful_frac": useful / (ms * 1e-3) / 1e12 / peak,
+            "executed_frac": executed / (ms * 1e-3) / 1e12 / peak,
+            "mfma_products": ALL_PAIRS_PRODUCTS[mode_name]}
+
+
+def all_pairs_subset_errors(X, modes, n_sub=2000):
+    """Max |GPU - FP64| (nm) of each mode on a seeded n_sub-frame subset of X,
+    and the FP64 oracle's time on the host cores."""
+    import numpy as np
+    import torch
+
+    from mdtraj_amd.allpairs import rmsd_matrix
+    from oracle import fp64
+
+    F = int(X.shape[0])
+    sub = np.sort(np.random.default_rng(2).choice(F, size=min(n_sub, F), replace=False))
+    Xs = X[torch.as_tensor(sub, device=X.device)].contiguous()
+    gpu = {m: rmsd_matrix(Xs, mode=m).cpu().numpy().astype(np.float64) for m in modes}
+    xs = Xs.cpu().numpy()
+    t0 = time.perf_counter()
+    want = fp64.condensed(fp64.rmsd_matrix(xs))
+    cpu_s = time.perf_counter() - t0
+    errs = {m: float(np.abs(v - want).max()) for m, v in gpu.items()}
+    log(f"all-pairs subset {len(sub)} frames: FP64 oracle {cpu_s:.1f} s; max err vs FP64 {errs}")
+    return errs, want.size, cpu_s, len(sub)
+
+
+def all_pairs_block(F=20_000, N=500, mode_name="f16exact"):
+    """The all-pairs workload (20,000 frames x 500 atoms, seeded random walk)
+    measured beside the one-vs-all line: the split-precision kernel and the
+    plain FP32 FMA kernel, each timed per launch, and both checked against
+    the FP64 oracle on a seeded 2,000-frame subset."""
+    import numpy as np
+    import torch
+
+    from mdtraj_amd import gen
+
+    X = gen.walk_device(seed=2, n_frames=F, n_atoms=N)
+    torch.cuda.synchronize()
+    n_pairs = F * (F - 1) // 2
+    split_ms, _, out = time_all_pairs(X, mode_name, warmup=3, steps=5)
+    del out
+    fp32_ms, _, out = time_all_pairs(X, "fp32", warmup=1, steps=2)
+    del out
+    torch.cuda.empty_cache()
+    split_med, fp32_med = float(np.median(split_ms)), float(np.median(fp32_ms))
+    errs, n_sub_pairs, cpu_s, n_sub = all_pairs_subset_errors(X, [mode_name, "fp32"])
+    fig = all_pairs_figures(F, N, mode_name, split_med)
+    log(f"all-pairs {F} x {N}: {mode_name} {split_med:.3f} ms, fp32 {fp32_med:.3f} ms")
+    return {
+        "workload": f"all-pairs RMSD matrix, {F} frames x {N} atoms (seeded random walk), "
+                    "condensed upper triangle",
+        "mode": mode_name,
+        "frame_pairs_per_s": n_pairs / (split_med * 1e-3),
+        "kernel_ms": split_med,
+        "kernel_ms_per_launch": [float(v) for v in split_ms],
+        "executed_mfma_frac": fig["executed_frac"],
+        "useful_mfma_frac": fig["useful_frac"],
+        "executed_tflops": fig["executed_tflops"],
+        "mfma_products": fig["mfma_products"],
+        "fp32_kernel_ms": fp32_med,
+        "fp32_frame_pairs_per_s": n_pairs / (fp32_med * 1e-3),
+        "speedup_vs_fp32": fp32_med / split_med,
+        "max_err_vs_fp64_nm": errs[mode_name],
+        "fp32_max_err_vs_fp64_nm": errs["fp32"],
+        "fp64_subset": f"seeded {n_sub}-frame subset, {n_sub_pairs} pairs, FP64 oracle {cpu_s:.1f} s",
+    }
+
+
+def bench_all_pairs(args):
+    import numpy as np
+    import torch
+
+    from mdtraj_amd import gen
+
+    F, N = args.frames, 500
+    log(f"generating walk trajectory {F} x {N} in HBM")
+    X = gen.walk_device(seed=2, n_frames=F, n_atoms=N)
+    torch.cuda.synchronize()
+    n_pairs = F * (F - 1) // 2
+    per, elapsed, out = time_all_pairs(X, args.mode, args.warmup, args.steps)
+    del out
+    ms = float(np.mean(per))
+    fig = all_pairs_figures(F, N, args.mode, ms)
+    log(f"all-pairs {args.mode}: {ms:.3f} ms, useful {fig['useful_tflops']:.1f} TFLOP/s, executed "
+        f"{fig['executed_tflops']:.1f} TFLOP/s")
     result = {
         "metric": METRIC,
         "value": args.steps * n_pairs / elapsed,
@@ -304,44 +396,21 @@ def bench_all_pairs(args):
                   "bf16exact": "bf16x4-exact-hi"}[args.mode],
         "data": "synthetic (seeded random walk generated in HBM)",
         "config": {"workload": f"all-pairs RMSD matrix, {F} frames x {N} atoms, mode {args.mode}",
-                   "kernel_ms": ms, "mfma_products": products,
-                   "executed_tflops": executed / (ms * 1e-3) / 1e12,
-                   "executed_frac": executed / (ms * 1e-3) / 1e12 / peak},
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": tflops / peak, "traffic": None},
+                   "kernel_ms": ms, "kernel_ms_per_launch": [float(v) for v in per],
+                   "mfma_products": fig["mfma_products"],
+                   "executed_tflops": fig["executed_tflops"],
+                   "executed_frac": fig["executed_frac"]},
+        "roofline": {"bound": "mfma", "achieved": fig["useful_tflops"], "peak": fig["peak_tflops"],
+                     "unit": "TFLOP/s", "frac": fig["useful_frac"], "traffic": None},
     }
-    del out, ws
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline_all_pairs(X, args.mode)
-    return result
-
-
-def cpu_baseline_all_pairs(X, mode_name, n_sub=2000):
-    """The FP64 oracle on a seeded 2,000-frame subset of the all-pairs
-    trajectory, timed on the host cores; on the same subset, the GPU's split
-    mode and the plain FP32 GPU kernel are checked against it."""
-    import numpy as np
-    import torch
-
-    from mdtraj_amd.allpairs import rmsd_matrix
-    from oracle import fp64
-
-    F = int(X.shape[0])
-    sub = np.sort(np.random.default_rng(2).choice(F, size=min(n_sub, F), replace=False))
-    Xs = X[torch.as_tensor(sub, device=X.device)].contiguous()
-    gpu = {m: rmsd_matrix(Xs, mode=m).cpu().numpy().astype(np.float64)
-           for m in sorted({mode_name, "fp32"})}
-    xs = Xs.cpu().numpy()
-    t0 = time.perf_counter()
-    want = fp64.condensed(fp64.rmsd_matrix(xs))
-    cpu_s = time.perf_counter() - t0
-    errs = {m: float(np.abs(v - want).max()) for m, v in gpu.items()}
-    log(f"all-pairs subset {len(sub)} frames: FP64 oracle {cpu_s:.1f} s; max err vs FP64 {errs}")
-    return {"value": want.size / cpu_s, "unit": "frame-pairs/s", "cores": cpu_threads(),
+        errs, n_sub_pairs, cpu_s, n_sub = all_pairs_subset_errors(X, sorted({args.mode, "fp32"}))
+        result["cpu_baseline"] = {
+            "value": n_sub_pairs / cpu_s, "unit": "frame-pairs/s", "cores": cpu_threads(),
             "kind": "port",
-            "sample": f"FP64 oracle (numpy) on a seeded {len(sub)}-frame subset, "
-                      f"{want.size} pairs",
+            "sample": f"FP64 oracle (numpy) on a seeded {n_sub}-frame subset, {n_sub_pairs} pairs",
             "max_err_vs_fp64_nm": errs}
+    return result
 
 
 # ------------------------------------------------------------------ batched

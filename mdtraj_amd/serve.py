@@ -62,10 +62,17 @@ class Request:
 
 
 class Pool:
-    def __init__(self, xyz, mode):
+    """A pool prepared on the caller's current stream.  ``ready`` is an event
+    recorded after the preparation: batches run on the server's own streams,
+    which are not ordered with the caller's, and wait on it before they read
+    the prepared block."""
+
+    def __init__(self, xyz, mode, torch_mod):
         self.n_frames, self.n_atoms = int(xyz.shape[0]), int(xyz.shape[1])
         self.split = SplitSet(xyz, mode=mode)
         self.nbytes = int(self.split.nbytes)
+        self.ready = torch_mod.cuda.Event()
+        self.ready.record()
 
 
 class RMSDServer:
@@ -111,11 +118,15 @@ class RMSDServer:
     # ---------------------------------------------------------------- pools
     def add_pool(self, xyz):
         """Prepare a pool (float32 (F, N, 3), host or device) in HBM; returns its id.
-        Raises MemoryError when the HBM budget cannot hold it."""
-        pool = Pool(dev.to_device_f32(xyz), self.mode)
+        Raises MemoryError when the HBM budget cannot hold it together with
+        one batch of ``max_request_queries`` query frames against it (so that
+        every request it may receive can run once nothing else is in flight)."""
+        pool = Pool(dev.to_device_f32(xyz), self.mode, self.t)
         with self.cv:
-            if self.resident + pool.nbytes > self.budget:
-                raise MemoryError(f"pool of {pool.nbytes} bytes exceeds the HBM budget")
+            one = self._batch_bytes(pool, self.max_req_q)
+            if self.resident + pool.nbytes + one > self.budget:
+                raise MemoryError(f"pool of {pool.nbytes} bytes plus one {self.max_req_q}-query "
+                                  f"batch ({one} bytes) exceeds the HBM budget")
             self.resident += pool.nbytes
             self.pools.append(pool)
             return len(self.pools) - 1
@@ -144,39 +155,61 @@ class RMSDServer:
         split = load().hr_split_bytes(nq, pool.n_atoms, MODES[self.mode])
         return 12 * nq * pool.n_atoms + int(split) + 4 * nq * pool.n_frames
 
-    def _take_batch(self):
+    def _take_batch(self, avail):
         """Oldest pending request plus later ones for the same pool (FIFO
-        within the pool), up to max_q query frames."""
+        within the pool), up to max_q query frames and `avail` bytes of HBM
+        in flight.  The oldest request must fit `avail`."""
         first = self.pending[0]
         pool = self.pools[first.pool]
         batch, nq = [], 0
         keep = collections.deque()
         while self.pending:
             r = self.pending.popleft()
-            if r.pool == first.pool and nq + r.queries.shape[0] <= self.max_q:
+            q = r.queries.shape[0]
+            if (r.pool == first.pool and nq + q <= self.max_q
+                    and (not batch or self._batch_bytes(pool, nq + q) <= avail)):
                 batch.append(r)
-                nq += r.queries.shape[0]
+                nq += q
             else:
                 keep.append(r)
         self.pending = keep
         return pool, batch, nq
 
+    def _wait_for_launch(self):
+        """Under self.cv: wait until the oldest pending request can be launched
+        (a free stream slot and HBM for it); False when stopped with nothing
+        pending.  Batches are sized by what they hold, so an admitted request
+        never waits for memory that no completion will free."""
+        while True:
+            if not self.pending:
+                if not self.running:
+                    return False
+                self.cv.wait()
+                continue
+            if not self.free_streams:
+                self.cv.wait()
+                continue
+            first = self.pending[0]
+            need = self._batch_bytes(self.pools[first.pool], first.queries.shape[0])
+            if self.resident + self.inflight_bytes + need <= self.budget:
+                return True
+            if self.inflight_bytes > 0:
+                self.cv.wait()     # a completion frees memory
+                continue
+            # cannot happen for pools admitted by add_pool; never wait forever
+            self.pending.popleft()
+            first._fail(MemoryError(f"a batch of {first.queries.shape[0]} query frames "
+                                    f"({need} bytes) does not fit the HBM budget"))
+
     def _scheduler(self):
         try:
             while True:
                 with self.cv:
-                    while self.running and (not self.pending or not self.free_streams):
-                        self.cv.wait()
-                    if not self.running and not self.pending:
+                    if not self._wait_for_launch():
                         return
-                    if not self.free_streams:
-                        continue
-                    pool = self.pools[self.pending[0].pool]
-                    need = self._batch_bytes(pool, self.max_q)
-                    while (self.running
-                           and self.resident + self.inflight_bytes + need > self.budget):
-                        self.cv.wait()
-                    pool, batch, nq = self._take_batch()
+                    avail = self.budget - self.resident - self.inflight_bytes
+                    pool, batch, nq = self._take_batch(avail)
+                    need = self._batch_bytes(pool, nq)
                     self.largest_batch = max(self.largest_batch, nq)
                     si = self.free_streams.popleft()
                     self.inflight_bytes += need
@@ -203,6 +236,7 @@ class RMSDServer:
         t = self.t
         stream = self.streams[si]
         queries = np.concatenate([r.queries for r in batch], axis=0)
+        stream.wait_event(pool.ready)
         with t.cuda.stream(stream):
             dq = t.from_numpy(queries).to(device=dev.device())
             qs = SplitSet(dq, mode=self.mode)

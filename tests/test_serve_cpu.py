@@ -16,7 +16,10 @@ from oracle import fp64
 
 class _Cuda:
     class Stream:
-        pass
+        waited = []
+
+        def wait_event(self, ev):
+            _Cuda.Stream.waited.append(ev)
 
     @staticmethod
     @contextlib.contextmanager
@@ -134,3 +137,54 @@ def test_failure_reaches_every_waiter(cpu_server, monkeypatch):
                 r.wait(30)
         with pytest.raises(RuntimeError):
             srv.submit(pid, pool[:1])
+
+
+def test_batches_wait_for_the_pool_preparation(cpu_server):
+    """Pools are prepared on the caller's stream and batches run on the
+    server's: every batch's stream waits on its pool's ready event first."""
+    _Cuda.Stream.waited.clear()
+    srv = cpu_server(hbm_budget=1e9)
+    pools = [gen.walk(seed=5 + i, n_frames=20, n_atoms=10) for i in range(2)]
+    ids = [srv.add_pool(p) for p in pools]
+    with srv:
+        for pid in ids:
+            srv.submit(pid, pools[pid][:2]).wait(30)
+    readies = [srv.pools[pid].ready for pid in ids]
+    assert len(_Cuda.Stream.waited) == srv.batches >= 2
+    assert all(any(w is r for r in readies) for w in _Cuda.Stream.waited)
+    assert any(w is readies[0] for w in _Cuda.Stream.waited)
+    assert any(w is readies[1] for w in _Cuda.Stream.waited)
+
+
+def test_pool_that_fits_but_not_with_a_full_batch(cpu_server):
+    """The budget holds the pool and one 64-query batch but not a full
+    256-query batch: requests still complete (batches are sized to what is
+    free), in several batches."""
+    pool = gen.walk(seed=6, n_frames=400, n_atoms=50)
+    probe = cpu_server(hbm_budget=1e12)
+    p = probe.add_pool(pool)
+    one = probe._batch_bytes(probe.pools[p], 64)
+    full = probe._batch_bytes(probe.pools[p], 256)
+    budget = probe.pools[p].nbytes + one + 1000
+    assert probe.pools[p].nbytes + full > budget
+    srv = cpu_server(hbm_budget=budget)
+    pid = srv.add_pool(pool)
+    reqs = [srv.submit(pid, pool[i:i + 40]) for i in range(0, 320, 40)]
+    with srv:
+        for i, r in zip(range(0, 320, 40), reqs):
+            got = r.wait(30)
+            want = np.asarray([fp64.rmsd(pool, pool[i:i + 40], k) for k in range(40)])
+            assert np.abs(got - want).max() < 1e-6
+    assert srv.largest_batch <= 64
+    assert srv.batches >= 5
+    assert srv.inflight_bytes == 0
+
+
+def test_budget_counts_one_request_batch(cpu_server):
+    """add_pool refuses a pool that fits only without room for one request."""
+    pool = gen.walk(seed=7, n_frames=300, n_atoms=40)
+    probe = cpu_server(hbm_budget=1e12)
+    p = probe.add_pool(pool)
+    srv = cpu_server(hbm_budget=probe.pools[p].nbytes + 10)
+    with pytest.raises(MemoryError, match="one 64-query batch"):
+        srv.add_pool(pool)
