@@ -74,13 +74,21 @@ int hr_device_cu_count(void);
  *                                  pass, per chunk); 0 the default (1024), a
  *                                  value at least the trajectory's size one
  *                                  fit pass and one rotation pass over all
- *                                  frames.  Results do not depend on it. */
+ *                                  frames.  Results do not depend on it.
+ *   HR_TUNE_HOST_DIRECT_KIB        hr_rmsd_one_vs_all_host: selections of up
+ *                                  to this many KiB in all (0: the default,
+ *                                  1024) are read by the kernels straight
+ *                                  from the pinned staging buffer and the
+ *                                  RMSDs written straight to pinned memory,
+ *                                  with no device copy either way; -1 never.
+ *                                  Results do not depend on it. */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2
 #define HR_TUNE_BATCH_SCHEDULE 3
 #define HR_TUNE_SUPERPOSE_CHUNK_MIB 4
-#define HR_TUNE_COUNT 5
+#define HR_TUNE_HOST_DIRECT_KIB 5
+#define HR_TUNE_COUNT 6
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 
@@ -415,6 +423,13 @@ int hr_dcd_decode(const void* data, int64_t n_bytes, const hr_dcd_header* header
                   int32_t n_sel, const int32_t* free_slot, float scale, float* xyz, float* cell,
                   int32_t* status, void* stream);
 
+/* Atom selection of a decoded device trajectory (the reader's copy of the
+ * selected atoms after decoding whole frames, formats/xtc/xtc.pyx:477-482):
+ * out (n_frames, n_sel, 3) = xyz (n_frames, n_atoms, 3) at atoms
+ * atom_idx[0..n_sel) (device int32; indices are clamped into range). */
+int hr_gather_atoms(const float* xyz, int64_t n_frames, int32_t n_atoms, const int32_t* atom_idx,
+                    int32_t n_sel, float* out, void* stream);
+
 /* ---------------------------------------------------------------------------
  * LP-RMSD: the RMSD minimised over rotations and over relabellings of atoms
  * inside permute groups
@@ -438,6 +453,57 @@ int hr_lprmsd(float* xyz, int64_t n_frames, int32_t n_atoms, const float* ref_fr
               const int32_t* atom_indices, int32_t n_sel, const int32_t* dis, int32_t n_dis,
               const int32_t* group_atoms, const int32_t* group_offsets, int32_t n_groups,
               int32_t max_group, int32_t superpose, float* out, void* workspace, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Memory, copies and streams, for callers without a device runtime of their
+ * own.  The reference's C ABI takes host pointers and allocates nothing
+ * (rmsd/include/theobald_rmsd.h:7-18); the device-pointer entry points above
+ * need device buffers, which these provide without HIP headers or torch.
+ *   hr_device_count   number of visible GPUs (0, and HR_OK, when none)
+ *   hr_alloc/hr_free  device memory (hipMalloc/hipFree; hr_free waits for
+ *                     the device, as hipFree does)
+ *   hr_alloc_host/hr_free_host  pinned host memory (hipHostMalloc)
+ *   hr_memcpy_*       copies on `stream` (NULL = the default stream):
+ *                     asynchronous between device and pinned memory; with
+ *                     pageable host memory they return when the copy is done
+ *   hr_memset         asynchronous byte fill on `stream`
+ *   hr_stream_*       a non-blocking stream of default priority
+ * ------------------------------------------------------------------------- */
+int hr_device_count(int* count);
+int hr_alloc(void** ptr, size_t nbytes);
+int hr_free(void* ptr);
+int hr_alloc_host(void** ptr, size_t nbytes);
+int hr_free_host(void* ptr);
+int hr_memcpy_h2d(void* dst, const void* src, size_t nbytes, void* stream);
+int hr_memcpy_d2h(void* dst, const void* src, size_t nbytes, void* stream);
+int hr_memcpy_d2d(void* dst, const void* src, size_t nbytes, void* stream);
+int hr_memset(void* dst, int value, size_t nbytes, void* stream);
+int hr_stream_create(void** stream);
+int hr_stream_destroy(void* stream);
+int hr_stream_synchronize(void* stream);
+
+/* ---------------------------------------------------------------------------
+ * One-vs-all RMSD over HOST memory (replaces the reference's host-pointer
+ * path: the gather of rmsd/_rmsd.pyx:197,199 and its frame loop
+ * rmsd/_rmsd.pyx:186-224 over msd_atom_major, rmsd/include/theobald_rmsd.h:
+ * 16-18).  Every pointer is host memory owned by the caller; the call is
+ * synchronous and returns when out holds the RMSDs.
+ *   target      (n_frames, n_atoms_total, 3) float32, not modified
+ *   atom_idx    int32[n_sel] into the target's atoms, or NULL (all atoms)
+ *   ref_frame   (n_ref_atoms_total, 3) float32
+ *   ref_idx     int32[n_sel] into the reference's atoms, or NULL (its first
+ *               n_sel atoms)
+ *   self_frame  target frame that IS the reference frame (exact 0), or -1
+ *   out         float32[n_frames]
+ * Indices are checked against the atom counts (HR_ERR_INVALID).  Only the
+ * selected atoms are copied to the device, in pinned chunks gathered by a
+ * worker pool while earlier chunks are copied and scored; staging buffers are
+ * kept per device between calls.  Calls on one device run one at a time.
+ * ------------------------------------------------------------------------- */
+int hr_rmsd_one_vs_all_host(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                            const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                            int32_t n_ref_atoms_total, const int32_t* ref_idx,
+                            int64_t self_frame, float* out);
 
 /* ---------------------------------------------------------------------------
  * Diagnostics (no counterpart in the reference): read `bytes` of device

@@ -22,6 +22,7 @@ HR_TUNE_FRAMES_GRID = 1
 HR_TUNE_FRAMES_ROW_LANES = 2
 HR_TUNE_BATCH_SCHEDULE = 3
 HR_TUNE_SUPERPOSE_CHUNK_MIB = 4
+HR_TUNE_HOST_DIRECT_KIB = 5
 
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1
@@ -64,6 +65,21 @@ _SIGNATURES = {
     "hr_batch_plan_launch": (ctypes.c_int, [_vp, _vp]),
     "hr_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "hr_batch_plan_units": (_i64, [_vp]),
+    "hr_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "hr_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), _sz]),
+    "hr_free": (ctypes.c_int, [_vp]),
+    "hr_alloc_host": (ctypes.c_int, [ctypes.POINTER(_vp), _sz]),
+    "hr_free_host": (ctypes.c_int, [_vp]),
+    "hr_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "hr_memcpy_d2h": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "hr_memcpy_d2d": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "hr_memset": (ctypes.c_int, [_vp, ctypes.c_int, _sz, _vp]),
+    "hr_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "hr_stream_destroy": (ctypes.c_int, [_vp]),
+    "hr_stream_synchronize": (ctypes.c_int, [_vp]),
+    "hr_rmsd_one_vs_all_host": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _i32, _vp, _i64,
+                                               _vp]),
+    "hr_gather_atoms": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp]),
     "hr_hbm_read_probe": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
     "hr_hbm_read_probe_nt": (ctypes.c_int, [_vp, _sz, _vp, ctypes.c_int, _vp]),
     "hr_xtc_scan": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp]),

@@ -55,6 +55,7 @@ def _validate_permute_groups(permute_groups, atom_indices):
     return groups
 
 
+@dev.follows(0, "target")
 def lprmsd(target, reference, frame=0, atom_indices=None, permute_groups=None, parallel=True,
            superpose=False):
     """LP-RMSD of every conformation in ``target`` to ``reference[frame]``
@@ -73,13 +74,13 @@ def lprmsd(target, reference, frame=0, atom_indices=None, permute_groups=None, p
     max_group = max([len(g) for g in rel], default=0)
 
     lib = load()
-    t = dev.torch()
     on_device = dev.is_device_array(txyz)
     # the reference frame is copied first (the reference copies it too), so a
     # target that shares its buffer is read unmoved
     R = dev.to_device_f32(rxyz[frame]).clone()
     if superpose and on_device:
-        if not (txyz.is_contiguous() and txyz.dtype == t.float32):
+        f32 = dev.torch().float32 if dev.is_torch_cuda(txyz) else np.dtype(np.float32)
+        if not (txyz.is_contiguous() and txyz.dtype == f32):
             raise ValueError("superpose needs target.xyz as a C-contiguous float32 tensor")
         T = txyz
     else:

@@ -39,13 +39,15 @@ def _xyz_of(obj):
 def _check_indices(idx, n_atoms, name):
     """ensure_type(np.asarray(idx), dtype=int, ndim=1) plus the range check of
     ``_rmsd.pyx:158-161,171-174``."""
-    if dev.is_device_array(idx):
+    if dev.is_torch_cuda(idx):
         a = idx
         if a.dim() != 1:
             raise ValueError(f"{name} must be ndim 1. You supplied {a.dim()}")
         if a.numel() and (int(a.min()) < 0 or int(a.max()) >= n_atoms):
             raise ValueError(f"{name} must be valid positive indices")
         return a
+    if isinstance(idx, dev.DeviceArray):
+        idx = idx.numpy()
     a = np.asarray(idx)
     if a.ndim != 1:
         raise ValueError(f"{name} must be ndim 1. You supplied {a.ndim}")
@@ -59,7 +61,8 @@ def _same_buffer(a, b) -> bool:
     if a is b:
         return True
     if dev.is_device_array(a) and dev.is_device_array(b):
-        return a.data_ptr() == b.data_ptr() and tuple(a.shape) == tuple(b.shape)
+        return (a.data_ptr() == b.data_ptr() and tuple(a.shape) == tuple(b.shape)
+                and type(a) is type(b))
     if isinstance(a, np.ndarray) and isinstance(b, np.ndarray):
         return (a.__array_interface__["data"][0] == b.__array_interface__["data"][0]
                 and a.shape == b.shape and a.strides == b.strides)
@@ -106,6 +109,29 @@ def _result(out, like_device):
     return out if like_device else dev.to_host(out)
 
 
+def _i32_host(idx):
+    return None if idx is None else np.ascontiguousarray(dev.indices_on_host(idx), dtype=np.int32)
+
+
+def _rmsd_host(txyz, rxyz, frame, idx, ridx, n_sel, same):
+    """Host arrays in, host array out: one synchronous call of
+    hr_rmsd_one_vs_all_host, which gathers the selection into pinned staging
+    chunks and scores each chunk on the GPU as it arrives."""
+    dev.require_gpu()
+    T = np.ascontiguousarray(txyz, dtype=np.float32)
+    R = np.ascontiguousarray(np.asarray(rxyz)[frame], dtype=np.float32)
+    n_frames, n_atoms = T.shape[0], T.shape[1]
+    out = np.empty(n_frames, np.float32)
+    if n_frames == 0:
+        return out
+    a, r = _i32_host(idx), _i32_host(ridx)
+    call("hr_rmsd_one_vs_all_host", T.ctypes.data, n_frames, n_atoms,
+         None if a is None else a.ctypes.data, n_sel, R.ctypes.data, R.shape[0],
+         None if r is None else r.ctypes.data, frame if same else -1, out.ctypes.data)
+    return out
+
+
+@dev.follows(0, "target")
 def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
          parallel=True, precentered=False, superpose=True):
     """RMSD of every conformation in ``target`` to ``reference[frame]``.
@@ -129,9 +155,11 @@ def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
     elif precentered:
         warnings.warn("in rmsd(), precentered is ignored when superpose=False", RuntimeWarning)
 
+    same = _same_buffer(txyz, rxyz) and ref_atom_indices is None
+    if superpose and isinstance(txyz, np.ndarray) and not dev.is_device_array(rxyz):
+        return _rmsd_host(txyz, rxyz, frame, idx, ridx, n_sel, same)
     lib = load()
     on_device = dev.is_device_array(txyz)
-    same = _same_buffer(txyz, rxyz) and ref_atom_indices is None
     T = dev.to_device_f32(txyz)
     if same:
         R = T[frame]
@@ -154,6 +182,7 @@ def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
     return _result(out, on_device)
 
 
+@dev.follows(0, "xyz")
 def _center_inplace_atom_major(xyz):
     """Centre (F, N, 3) float32 coordinates in place and return the traces.
 
@@ -162,7 +191,8 @@ def _center_inplace_atom_major(xyz):
     """
     if dev.is_device_array(xyz):
         assert xyz.dim() == 3 and xyz.shape[2] == 3
-        if not xyz.is_contiguous() or xyz.dtype != dev.torch().float32:
+        f32 = dev.torch().float32 if dev.is_torch_cuda(xyz) else np.dtype(np.float32)
+        if not xyz.is_contiguous() or xyz.dtype != f32:
             raise ValueError("xyz must be a C-contiguous float32 tensor")
         traces = dev.empty_f32(xyz.shape[0])
         call("hr_center_and_trace", xyz.data_ptr(), traces.data_ptr(), int(xyz.shape[0]),
@@ -182,6 +212,7 @@ def _center_inplace_atom_major(xyz):
     return dev.to_host(traces)
 
 
+@dev.follows(1, "xyz2")
 def getMultipleRMSDs_atom_major(xyz1, xyz2, g1, g2, frame, parallel=True):
     """RMSDs between xyz1[frame] and every frame of xyz2 (atom-major).
 
@@ -208,6 +239,7 @@ def getMultipleRMSDs_atom_major(xyz1, xyz2, g1, g2, frame, parallel=True):
     return _result(out, dev.is_device_array(xyz2))
 
 
+@dev.follows(1, "xyz2")
 def getMultipleRMSDs_axis_major(xyz1, xyz2, g1, g2, frame, parallel=True):
     """RMSDs between xyz1[frame] and every frame of xyz2, axis-major
     (n_frames, 3, n_atoms).  Mirrors ``rmsd/_rmsd.pyx:502-557``."""
@@ -230,6 +262,7 @@ def getMultipleRMSDs_axis_major(xyz1, xyz2, g1, g2, frame, parallel=True):
     return _result(out, dev.is_device_array(xyz2))
 
 
+@dev.follows(4, "xyz_displace_mobile")
 def superpose_atom_major(xyz_align_target, xyz_align_mobile, g_target, g_mobile,
                          xyz_displace_mobile, target_frame, parallel=True):
     """Rotate every frame of ``xyz_displace_mobile`` in place by the rotation
@@ -258,6 +291,7 @@ def superpose_atom_major(xyz_align_target, xyz_align_mobile, g_target, g_mobile,
         xyz_displace_mobile.copy_(D)
 
 
+@dev.follows(0, "target")
 def rmsf(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
          parallel=True, precentered=False, mode="atom"):
     """Root-mean-square fluctuation of the selected atoms of ``target``.
@@ -346,6 +380,7 @@ def rmsf(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
     return res.astype(np.float32)
 
 
+@dev.follows(1, "xyz_align2")
 def getMultipleAlignDisplaceRMSDs_atom_major(xyz_align1, xyz_align2, g_align1, g_align2,
                                              xyz_displ1, xyz_displ2, n_atoms_align,
                                              n_atoms_displ, frame, parallel=True):

@@ -19,6 +19,7 @@ MODES = {"fp32": HR_PAIRS_FP32, "f16x3": HR_PAIRS_F16X3, "bf16x3": HR_PAIRS_BF16
          "f16exact": HR_PAIRS_F16_EXACT, "bf16exact": HR_PAIRS_BF16_EXACT}
 
 
+@dev.follows(0, "traj")
 def rmsd_matrix(traj, atom_indices=None, mode="f16exact", square=False, out=None):
     """RMSD between every pair of frames of ``traj``.
 
@@ -52,12 +53,17 @@ def rmsd_matrix(traj, atom_indices=None, mode="f16exact", square=False, out=None
          out.data_ptr(), ws.data_ptr(), nbytes, dev.stream())
     out = out[:n_pairs]
     if square:
-        t = dev.torch()
-        D = t.zeros((F, F), dtype=t.float32, device=out.device)
-        iu = t.triu_indices(F, F, offset=1, device=out.device)
-        D[iu[0], iu[1]] = out
+        if dev.is_torch_cuda(out):
+            t = dev.torch()
+            D = t.zeros((F, F), dtype=t.float32, device=out.device)
+            iu = t.triu_indices(F, F, offset=1, device=out.device)
+            D[iu[0], iu[1]] = out
+            D = D + D.T
+            return D if on_device else dev.to_host(D)
+        D = np.zeros((F, F), np.float32)
+        D[np.triu_indices(F, k=1)] = dev.to_host(out)
         D = D + D.T
-        return D if on_device else dev.to_host(D)
+        return dev.to_device_f32(D) if on_device else D
     return out if on_device else dev.to_host(out)
 
 
@@ -68,8 +74,12 @@ class SplitSet:
     def __init__(self, traj, atom_indices=None, mode="f16exact"):
         if mode not in MODES or mode == "fp32":
             raise ValueError("SplitSet needs a split mode")
-        lib = load()
         xyz = traj.xyz if hasattr(traj, "xyz") else traj
+        with dev.using(xyz):
+            self._prepare(xyz, atom_indices, mode)
+
+    def _prepare(self, xyz, atom_indices, mode):
+        lib = load()
         X = dev.to_device_f32(xyz)
         self.n_frames, n_atoms = int(X.shape[0]), int(X.shape[1])
         if atom_indices is not None:
@@ -90,11 +100,11 @@ def rmsd_cross(a, b, out=None):
     (columns): float32 CUDA tensor (a.n_frames, b.n_frames)."""
     if a.mode != b.mode or a.n_sel != b.n_sel:
         raise ValueError("sets must share the mode and the number of selected atoms")
-    t = dev.torch()
-    if out is None:
-        out = t.empty((a.n_frames, b.n_frames), dtype=t.float32, device=a.block.device)
-    call("hr_rmsd_cross", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
-         a.n_sel, MODES[a.mode], out.data_ptr(), out.stride(0), dev.stream())
+    with dev.using(a.block):
+        if out is None:
+            out = dev.empty((a.n_frames, b.n_frames), np.float32)
+        call("hr_rmsd_cross", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
+             a.n_sel, MODES[a.mode], out.data_ptr(), out.stride(0), dev.stream())
     return out
 
 
@@ -109,12 +119,13 @@ def nearest(a, b):
     if b.n_frames == 0 and a.n_frames:
         raise ValueError("no frames to take the nearest of")
     lib = load()
-    t = dev.torch()
-    index = t.empty(a.n_frames, dtype=t.int32, device=a.block.device)
-    value = t.empty(a.n_frames, dtype=t.float32, device=a.block.device)
-    ws = dev.workspace(lib.hr_rmsd_cross_argmin_workspace_bytes(a.n_frames))
-    call("hr_rmsd_cross_argmin", a.block.data_ptr(), a.n_frames, b.block.data_ptr(), b.n_frames,
-         a.n_sel, MODES[a.mode], index.data_ptr(), value.data_ptr(), ws.data_ptr(), dev.stream())
+    with dev.using(a.block):
+        index = dev.empty(a.n_frames, np.int32)
+        value = dev.empty(a.n_frames, np.float32)
+        ws = dev.workspace(lib.hr_rmsd_cross_argmin_workspace_bytes(a.n_frames))
+        call("hr_rmsd_cross_argmin", a.block.data_ptr(), a.n_frames, b.block.data_ptr(),
+             b.n_frames, a.n_sel, MODES[a.mode], index.data_ptr(), value.data_ptr(),
+             ws.data_ptr(), dev.stream())
     return index, value
 
 
@@ -125,12 +136,12 @@ def assign_to_leaders(frames, leaders, atom_indices=None, mode="f16exact"):
     return dev.to_host(idx), dev.to_host(val)
 
 
+@dev.follows(0, "condensed")
 def centroid_scores(condensed, n_frames, beta=1.0):
     """``sum_j exp(-beta d_ij / d.std())`` for every frame i of the full
     symmetric matrix whose condensed upper triangle is ``condensed``
     (examples/centroids.ipynb, cells 3-4); float64 numpy array."""
     lib = load()
-    t = dev.torch()
     n = int(n_frames)
     if n < 0:
         raise ValueError("n_frames must be >= 0")
@@ -139,13 +150,14 @@ def centroid_scores(condensed, n_frames, beta=1.0):
         D = D.clone()
     if D.numel() != n * (n - 1) // 2:
         raise ValueError(f"condensed length {D.numel()} does not match {n} frames")
-    scores = t.empty(max(n, 1), dtype=t.float64, device=D.device)
+    scores = dev.empty(max(n, 1), np.float64)
     ws = dev.workspace(lib.hr_centroid_workspace_bytes(n))
     call("hr_centroid_scores", D.data_ptr() if D.numel() else None, n, float(beta),
          scores.data_ptr(), None, ws.data_ptr(), dev.stream())
     return dev.to_host(scores[:n])
 
 
+@dev.follows(0, "traj")
 def centroid(traj, atom_indices=None, beta=1.0, mode="f16exact"):
     """Index of the frame with the largest summed similarity to all others
     (examples/centroids.ipynb): all-pairs RMSD, then centroid_scores."""

@@ -22,14 +22,19 @@ class BatchRMSD:
     ``unit_kib`` (if > 0, fewer frames per unit, a multiple of 4, where
     frames_per_unit frames would exceed that many KiB) and ``waves`` (3 or 4
     waves per SIMD of the scoring kernel); None picks the autotuned value for
-    this batch (``mdtraj_amd.autotune``).  ``run()`` is one graph launch on
-    torch's current stream and returns the list of float32 CUDA tensors of
-    RMSDs.
+    this batch (``mdtraj_amd.autotune``, which times with torch events).
+    ``run()`` is one graph launch and returns the list of float32 device
+    arrays of RMSDs: CUDA tensors on torch's current stream for tensor
+    systems, DeviceArrays on the default stream otherwise.
     """
 
     def __init__(self, systems, ref_frames=None, frames_per_unit=None, waves=None,
                  unit_kib=None):
         load()
+        with dev.using(*systems):
+            self._init(systems, ref_frames, frames_per_unit, waves, unit_kib)
+
+    def _init(self, systems, ref_frames, frames_per_unit, waves, unit_kib):
         self.xyz = [dev.to_device_f32(x) for x in systems]
         for i, x in enumerate(self.xyz):
             if x.dim() != 3 or x.shape[2] != 3:
@@ -82,9 +87,10 @@ class BatchRMSD:
         return sum(12 * int(x.shape[0]) * int(x.shape[1]) for x in self.xyz)
 
     def run(self):
-        """One graph launch on torch's current stream; returns the per-system
-        float32 device views of the results (the same tensors every call)."""
-        call("hr_batch_plan_launch", self._plan, dev.stream())
+        """One graph launch; returns the per-system float32 device views of
+        the results (the same arrays every call)."""
+        with dev.using(self.xyz[0]):
+            call("hr_batch_plan_launch", self._plan, dev.stream())
         return list(self._views)
 
     def close(self):

@@ -162,22 +162,24 @@ class DCDTrajectoryFile:
             n_sel = sel.size
         step = 1 if stride is None else max(1, int(stride))
         frames = np.arange(self.tell(), len(self), step, dtype=np.int64)[:count]
-        t = dev.torch()
-        d = dev.device()
+        with dev.using_loaded():
+            return self._decode(frames, sel, n_sel, step, scale)
+
+    def _decode(self, frames, sel, n_sel, step, scale):
         F = int(frames.size)
-        xyz = t.empty((F, n_sel, 3), dtype=t.float32, device=d)
+        xyz = dev.empty((F, n_sel, 3), np.float32)
         cell = np.zeros((0, 6), np.float32)
         if F and n_sel:
             data, slot = self._device_bytes()
-            idx = None if sel is None else t.from_numpy(sel.astype(np.int32)).to(d)
-            fr = t.from_numpy(frames).to(d)
-            cell_d = t.empty((F, 6), dtype=t.float32, device=d)
-            status = t.empty(F, dtype=t.int32, device=d)
+            idx = None if sel is None else dev.to_device_i32(sel.astype(np.int32))
+            fr = dev.to_device(frames, np.int64)
+            cell_d = dev.empty((F, 6), np.float32)
+            status = dev.empty(F, np.int32)
             call("hr_dcd_decode", data.data_ptr(), data.numel(), ctypes.byref(self._header),
                  fr.data_ptr(), F, dev.ptr(idx), n_sel, dev.ptr(slot), float(scale),
                  xyz.data_ptr(), cell_d.data_ptr(), status.data_ptr(), dev.stream())
-            st = status.cpu().numpy()
-            cell = cell_d.cpu().numpy()
+            st = dev.to_host(status)
+            cell = dev.to_host(cell_d)
             bad = np.flatnonzero(st)
             if bad.size:
                 b = int(bad[0])
@@ -195,16 +197,15 @@ class DCDTrajectoryFile:
         return xyz, lengths, angles
 
     def _device_bytes(self):
-        if self._dev is None:
-            t = dev.torch()
-            self._dev = t.from_numpy(self._host).to(dev.device())
+        if self._dev is None or dev.is_torch_cuda(self._dev) != dev.is_torch_mode():
+            self._dev = dev.to_device(self._host, np.uint8)
             if self._free is not None:
                 # an atom listed twice takes its last slot, as the reference's copy loop
                 rev = self._free[::-1].astype(np.int64) - 1
                 atoms, first_in_rev = np.unique(rev, return_index=True)
                 slot = np.full(self.n_atoms, -1, np.int32)
                 slot[atoms] = (self._free.size - 1 - first_in_rev).astype(np.int32)
-                self._dev_slot = t.from_numpy(slot).to(dev.device())
+                self._dev_slot = dev.to_device_i32(slot)
         return self._dev, self._dev_slot
 
 

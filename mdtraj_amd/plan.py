@@ -2,8 +2,8 @@
 
 ``OneVsAll`` holds a device trajectory, its atom selection and a workspace;
 each ``run`` scores every frame against one reference conformation with two
-kernel launches (reference prep + per-frame kernel) on torch's current
-stream.  This is what repeated calls (clustering loops, serving, benchmarks)
+kernel launches (reference prep + per-frame kernel): on torch's current
+stream for a tensor trajectory, else on the default stream.  This is what repeated calls (clustering loops, serving, benchmarks)
 use instead of ``md.rmsd``, whose per-call work is host validation and
 transfers.
 """
@@ -17,6 +17,10 @@ from ._lib import call, load
 
 class OneVsAll:
     def __init__(self, target_xyz, atom_indices=None):
+        with dev.using(target_xyz):
+            self._init(target_xyz, atom_indices)
+
+    def _init(self, target_xyz, atom_indices):
         lib = load()
         self.xyz = dev.to_device_f32(target_xyz)
         if self.xyz.dim() != 3 or self.xyz.shape[2] != 3:
@@ -34,6 +38,10 @@ class OneVsAll:
 
     def set_reference(self, ref_frame_xyz, ref_atom_indices=None):
         """Gather and sum the reference conformation (N_ref, 3) on the device."""
+        with dev.using(self.xyz):
+            return self._set_reference(ref_frame_xyz, ref_atom_indices)
+
+    def _set_reference(self, ref_frame_xyz, ref_atom_indices):
         R = dev.to_device_f32(ref_frame_xyz)
         if ref_atom_indices is None and self.idx is not None:
             ref_atom_indices = self.idx
@@ -53,8 +61,10 @@ class OneVsAll:
         the float32 device tensor of RMSDs."""
         o = self.out if out is None else out
         sf = getattr(self, "_self_frame", -1) if self_frame is None else self_frame
-        call("hr_rmsd_one_vs_all_prepared", self.xyz.data_ptr(), self.n_frames, self.n_atoms,
-             dev.ptr(self.idx), self.n_sel, sf, o.data_ptr(), self.ws.data_ptr(), dev.stream())
+        with dev.using(self.xyz):
+            call("hr_rmsd_one_vs_all_prepared", self.xyz.data_ptr(), self.n_frames, self.n_atoms,
+                 dev.ptr(self.idx), self.n_sel, sf, o.data_ptr(), self.ws.data_ptr(),
+                 dev.stream())
         return o
 
     def __call__(self, frame):

@@ -88,7 +88,7 @@ class RMSDServer:
 
     def __init__(self, hbm_budget=200e9, mode="f16exact", max_batch_queries=256,
                  max_request_queries=64, n_streams=3, keep_results=True):
-        self.t = dev.require_gpu()
+        self.t = dev.require_torch_gpu()
         self.mode = mode
         self.budget = int(hbm_budget)
         self.max_q = int(max_batch_queries)
@@ -121,7 +121,8 @@ class RMSDServer:
         Raises MemoryError when the HBM budget cannot hold it together with
         one batch of ``max_request_queries`` query frames against it (so that
         every request it may receive can run once nothing else is in flight)."""
-        pool = Pool(dev.to_device_f32(xyz), self.mode, self.t)
+        with dev.torch_buffers():
+            pool = Pool(dev.to_device_f32(xyz), self.mode, self.t)
         with self.cv:
             one = self._batch_bytes(pool, self.max_req_q)
             if self.resident + pool.nbytes + one > self.budget:
@@ -237,7 +238,7 @@ class RMSDServer:
         stream = self.streams[si]
         queries = np.concatenate([r.queries for r in batch], axis=0)
         stream.wait_event(pool.ready)
-        with t.cuda.stream(stream):
+        with t.cuda.stream(stream), dev.torch_buffers():
             dq = t.from_numpy(queries).to(device=dev.device())
             qs = SplitSet(dq, mode=self.mode)
             out = rmsd_cross(qs, pool.split)

@@ -8,8 +8,9 @@ float32, C-contiguous, resets the trace cache on assignment), the
 cache (``:1328-1359``).  Topology, I/O and the other analyses are outside this
 package; ``topology`` is stored as given.
 
-``xyz`` may be a numpy array (host) or a float32 CUDA tensor (device
-resident: the RMSD calls then read it in place from HBM, with no transfer).
+``xyz`` may be a numpy array (host), a ``DeviceArray`` or a float32 CUDA
+tensor (device resident: the RMSD calls then read it in place from HBM, with
+no transfer).
 """
 from __future__ import annotations
 
@@ -43,13 +44,11 @@ class Trajectory:
         """float32 C-contiguous (n_frames, n_atoms, 3); a 2-D array gains a
         frame axis (``ensure_type(..., add_newaxis_on_deficient_ndim=True)``)."""
         if dev.is_device_array(value):
-            t = dev.torch()
             if value.dim() == 2:
-                value = value.unsqueeze(0)
+                value = value.reshape(1, *value.shape)
             if value.dim() != 3 or value.shape[2] != 3:
                 raise ValueError(f"xyz must be shape (n_frames, n_atoms, 3), got {tuple(value.shape)}")
-            if value.dtype != t.float32 or not value.is_contiguous():
-                value = value.to(t.float32).contiguous()
+            value = dev.to_device_f32(value)
         else:
             value = np.asarray(value)
             if value.ndim == 2:
@@ -139,6 +138,10 @@ class Trajectory:
         if ref_atom_indices is not None and len(ref_atom_indices) != len(atom_indices):
             raise ValueError("Number of atoms must be consistent!")
 
+        with dev.using(self._xyz):
+            return self._superpose(reference, frame, atom_indices, ref_atom_indices)
+
+    def _superpose(self, reference, frame, atom_indices, ref_atom_indices):
         lib = load()
         rxyz = reference.xyz if hasattr(reference, "xyz") else reference
         on_device = self.is_on_device
@@ -150,21 +153,23 @@ class Trajectory:
         R = dev.to_device_f32(rxyz[frame])
         n_frames, n_atoms = int(X.shape[0]), int(X.shape[1])
         n_sel = n_atoms if atom_indices is None else len(atom_indices)
-        idx = dev.to_device_i32(None if atom_indices is None else np.asarray(atom_indices))
-        ridx = dev.to_device_i32(None if ref_atom_indices is None else np.asarray(ref_atom_indices))
-        if idx is not None and (int(idx.min()) < 0 or int(idx.max()) >= n_atoms):
+        a = None if atom_indices is None else dev.indices_on_host(atom_indices)
+        r = None if ref_atom_indices is None else dev.indices_on_host(ref_atom_indices)
+        if a is not None and a.size and (a.min() < 0 or a.max() >= n_atoms):
             raise IndexError("atom_indices out of range")
-        if ridx is not None and (int(ridx.min()) < 0 or int(ridx.max()) >= int(R.shape[0])):
+        if r is not None and r.size and (r.min() < 0 or r.max() >= int(R.shape[0])):
             raise IndexError("ref_atom_indices out of range")
         if atom_indices is None and int(R.shape[0]) != n_atoms:
             raise ValueError("Number of atoms must be consistent!")
+        idx = dev.to_device_i32(a)
+        ridx = dev.to_device_i32(r)
         if n_frames:
             ws = dev.workspace(lib.hr_superpose_workspace_bytes(n_frames, n_sel))
             call("hr_superpose", X.data_ptr(), n_frames, n_atoms, None, n_atoms, dev.ptr(idx),
                  n_sel, R.data_ptr(), dev.ptr(ridx), 1, None, None, ws.data_ptr(), dev.stream())
             # core/trajectory.py:1162-1169: an all-zero first frame means the
             # rotation overflowed or underflowed.
-            if not bool(X[0].any()):
+            if not dev.any_nonzero(X[0]):
                 raise OverflowError(
                     "Encounted a potential overflow/underflow error during superpose() due to "
                     "the magnitude of your `_xyz` coordinates. To circumvent this, (1) reload "

@@ -111,14 +111,12 @@ class XTCTrajectoryFile:
             if atom_indices.size and atom_indices.max() >= self.n_atoms:
                 raise ValueError("atom indices should be zero indexed. you gave an index bigger "
                                  "than the number of atoms")
-        xyz, time, step, box = self._decode(frames)
-        if atom_indices is not None:
-            t = dev.torch()
-            if isinstance(atom_indices, slice):
-                xyz = xyz[:, atom_indices].contiguous()
-            else:
-                xyz = xyz.index_select(1, t.as_tensor(atom_indices, dtype=t.int64,
-                                                      device=xyz.device)).contiguous()
+        with dev.using_loaded():
+            xyz, time, step, box = self._decode(frames)
+            if atom_indices is not None:
+                sel = (np.arange(self.n_atoms)[atom_indices] if isinstance(atom_indices, slice)
+                       else atom_indices)
+                xyz = self._select_atoms(xyz, sel)
         if frames.size:
             self.frame_counter = int(min(len(self), frames[-1] + stride))
         if np.all(np.logical_and(box < 1e-10, box > -1e-10)):
@@ -127,35 +125,44 @@ class XTCTrajectoryFile:
 
     # -- device decode -------------------------------------------------------
     def _device_bytes(self):
-        if self._dev is None:
-            t = dev.torch()
-            self._dev = t.from_numpy(self._host).to(dev.device())
+        if self._dev is None or dev.is_torch_cuda(self._dev) != dev.is_torch_mode():
+            self._dev = dev.to_device(self._host, np.uint8)
         return self._dev
 
+    @staticmethod
+    def _select_atoms(xyz, sel):
+        """(F, n_sel, 3) device copy of the atoms `sel` of every frame."""
+        sel = np.ascontiguousarray(sel, dtype=np.int32)
+        F, N = int(xyz.shape[0]), int(xyz.shape[1])
+        out = dev.empty((F, sel.size, 3), np.float32)
+        if F and sel.size:
+            idx = dev.to_device_i32(sel)
+            call("hr_gather_atoms", xyz.data_ptr(), F, N, idx.data_ptr(), int(sel.size),
+                 out.data_ptr(), dev.stream())
+        return out
+
     def _decode(self, frames):
-        t = dev.torch()
         F, N = int(frames.size), self.n_atoms
-        d = dev.device()
-        xyz = t.empty((F, N, 3), dtype=t.float32, device=d)
-        time = t.empty(F, dtype=t.float32, device=d)
-        step = t.empty(F, dtype=t.int32, device=d)
-        box = t.empty((F, 3, 3), dtype=t.float32, device=d)
-        prec = t.empty(F, dtype=t.float32, device=d)
-        status = t.empty(F, dtype=t.int32, device=d)
+        xyz = dev.empty((F, N, 3), np.float32)
+        time = dev.empty(F, np.float32)
+        step = dev.empty(F, np.int32)
+        box = dev.empty((F, 3, 3), np.float32)
+        prec = dev.empty(F, np.float32)
+        status = dev.empty(F, np.int32)
         if F:
             data = self._device_bytes()
-            offs = t.from_numpy(self._offsets[frames]).to(d)
+            offs = dev.to_device(self._offsets[frames], np.int64)
             call("hr_xtc_decode", data.data_ptr(), data.numel(), offs.data_ptr(), F, N,
                  xyz.data_ptr(), time.data_ptr(), step.data_ptr(), box.data_ptr(),
                  prec.data_ptr(), status.data_ptr(), dev.stream())
-            st = status.cpu().numpy()
+            st = dev.to_host(status)
             bad = np.flatnonzero(st)
             if bad.size:
                 f = int(frames[bad[0]])
                 raise RuntimeError(f"XTC read error: {_STATUS.get(int(st[bad[0]]), 'unknown')} "
                                    f"(frame {f})")
-        self.last_precision = prec.cpu().numpy()
-        return xyz, time.cpu().numpy(), step.cpu().numpy(), box.cpu().numpy()
+        self.last_precision = dev.to_host(prec)
+        return xyz, dev.to_host(time), dev.to_host(step), dev.to_host(box)
 
 
 def load_xtc(filename, top=None, stride=None, atom_indices=None, frame=None):

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from mdtraj_amd import gen
+from mdtraj_amd import _device as dev
 from mdtraj_amd.batch import BatchRMSD
 from oracle import fp64
 
@@ -60,7 +61,7 @@ def test_batch_graph_replays_are_identical(gpu):
         again = b.run()
         torch.cuda.synchronize()
         for a, f in zip(again, first):
-            assert torch.equal(a, f)
+            assert np.array_equal(dev.to_host(a), dev.to_host(f))
         del junk
     b.close()
 
@@ -74,4 +75,4 @@ def test_autotuner_candidates_agree(gpu):
     a = BatchRMSD(xs, frames_per_unit=16, unit_kib=0, waves=3).run()
     b = BatchRMSD(xs, frames_per_unit=64, unit_kib=32, waves=4).run()
     for u, v in zip(a, b):
-        assert gpu.equal(u, v)
+        assert np.array_equal(dev.to_host(u), dev.to_host(v))

@@ -63,7 +63,7 @@ def _cross(qs, ps):
 
 @pytest.fixture
 def cpu_server(monkeypatch):
-    monkeypatch.setattr(serve.dev, "require_gpu", lambda: _Torch())
+    monkeypatch.setattr(serve.dev, "require_torch_gpu", lambda: _Torch())
     monkeypatch.setattr(serve.dev, "device", lambda: "cpu")
     monkeypatch.setattr(serve.dev, "to_device_f32",
                         lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)))
