@@ -502,8 +502,14 @@ def bench_batched(args):
     from mdtraj_amd import autotune, gen
     from mdtraj_amd.batch import BatchRMSD
 
-    log("generating 512 uneven systems in HBM")
-    xs = gen.uneven_systems_device(seed=3)
+    if args.host_gen:
+        # numpy generation and one upload per system: no device RNG kernels
+        # in the process (for counter-collection runs)
+        log("generating 512 uneven systems on the host, then uploading")
+        xs = [torch.from_numpy(x).to("cuda") for x in gen.uneven_systems(seed=3)]
+    else:
+        log("generating 512 uneven systems in HBM")
+        xs = gen.uneven_systems_device(seed=3)
     torch.cuda.synchronize()
     if args.batch_shape:
         best = tuple(int(v) for v in args.batch_shape.split(","))
@@ -1137,12 +1143,17 @@ def main():
     ap.add_argument("--batch-shape", default=None,
                     help="batched workload: frames_per_unit,unit_kib,waves_per_simd instead of "
                          "autotuning (e.g. 64,192,3), so that a profile holds only that shape")
+    ap.add_argument("--host-gen", action="store_true",
+                    help="batched workload: generate the systems with numpy on the host")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--frames-waves", type=int, default=0, choices=[0, 3, 4],
                     help="register budget of the per-frame kernel (0 = library default)")
     ap.add_argument("--frames-grid", type=int, default=0, choices=[0, 1, 2],
                     help="grid of the per-frame kernel: 1 equal quad counts, 2 one wave "
                          "per resident slot (0 = library default, 2)")
+    ap.add_argument("--pairs-tile", type=int, default=0, choices=[0, 1, 2],
+                    help="split-precision Gram kernels: 1 64x64 tiles, 2 64x32 half tiles "
+                         "(0 = library default, 2)")
     ap.add_argument("--frames-row-lanes", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
                     help="lanes per frame of the per-frame kernel (0 = library default)")
     args = ap.parse_args()
@@ -1169,6 +1180,7 @@ def main():
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, args.frames_waves)
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_GRID, args.frames_grid)
     _lib.call("hr_set_tuning", _lib.HR_TUNE_FRAMES_ROW_LANES, args.frames_row_lanes)
+    _lib.call("hr_set_tuning", _lib.HR_TUNE_PAIRS_TILE, args.pairs_tile)
     if args.workload == "hbm-ceiling":
         print(json.dumps(bench_hbm_ceiling(args)), flush=True)
         return

@@ -81,14 +81,21 @@ int hr_device_cu_count(void);
  *                                  from the pinned staging buffer and the
  *                                  RMSDs written straight to pinned memory,
  *                                  with no device copy either way; -1 never.
- *                                  Results do not depend on it. */
+ *                                  Results do not depend on it.
+ *   HR_TUNE_PAIRS_TILE             split-precision Gram kernels (all-pairs,
+ *                                  cross, nearest column): 1 64 x 64 pair
+ *                                  tiles, 8 waves, one workgroup per CU;
+ *                                  2 (default) 64 x 32 tiles, 4 waves, two
+ *                                  workgroups per CU.  Results do not depend
+ *                                  on it. */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2
 #define HR_TUNE_BATCH_SCHEDULE 3
 #define HR_TUNE_SUPERPOSE_CHUNK_MIB 4
 #define HR_TUNE_HOST_DIRECT_KIB 5
-#define HR_TUNE_COUNT 6
+#define HR_TUNE_PAIRS_TILE 6
+#define HR_TUNE_COUNT 7
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 

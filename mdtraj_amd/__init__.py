@@ -24,7 +24,7 @@ import os
 
 from . import _rmsd
 from ._lprmsd import lprmsd
-from ._rmsd import rmsd, rmsf
+from ._rmsd import rmsd, rmsf, set_inplace_centering
 from .trajectory import Trajectory
 
 __version__ = "0.1.0"
@@ -44,4 +44,5 @@ def rmsd_matrix(*args, **kwargs):
     return _rm(*args, **kwargs)
 
 
-__all__ = ["rmsd", "rmsf", "lprmsd", "Trajectory", "capi", "rmsd_matrix", "_rmsd"]
+__all__ = ["rmsd", "rmsf", "lprmsd", "Trajectory", "capi", "rmsd_matrix", "_rmsd",
+           "set_inplace_centering"]

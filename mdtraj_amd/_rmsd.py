@@ -21,6 +21,13 @@ Documented deviations from the reference:
   given, ``ValueError`` is raised (the reference's check there calls
   ``len(slice)`` and raises ``TypeError``).
 * device-resident inputs (torch CUDA tensors as ``.xyz``) return a CUDA tensor.
+
+``set_inplace_centering(True)`` restores the reference's side effect for
+callers that depend on it (tests/test_rmsd_memmap.py): with ``superpose`` and
+no ``atom_indices``, ``rmsd`` centres ``target.xyz`` and ``reference.xyz[frame]``
+in place (bit-exact to the reference's float32 centring) unless cached traces
+are used, and a read-only ``xyz`` raises ``ValueError`` as the reference's
+memoryview does.
 """
 from __future__ import annotations
 
@@ -30,6 +37,19 @@ import numpy as np
 
 from . import _device as dev
 from ._lib import call, load
+
+
+_INPLACE = False
+
+
+def set_inplace_centering(enabled):
+    """Opt in (True) or out (False, the default) of the reference's in-place
+    centring of ``target.xyz`` and ``reference.xyz[frame]`` by ``rmsd``
+    (rmsd/_rmsd.pyx:197,199,212-213, reached when ``atom_indices`` is None).
+    Returns the previous setting."""
+    global _INPLACE
+    prev, _INPLACE = _INPLACE, bool(enabled)
+    return prev
 
 
 def _xyz_of(obj):
@@ -109,6 +129,17 @@ def _result(out, like_device):
     return out if like_device else dev.to_host(out)
 
 
+def _centre_like_reference(txyz, rxyz, frame):
+    """The reference's side effect on float32 C-ordered host arrays: every
+    target frame centred, then the reference frame (a view, so already
+    centred when target is reference) centred again (rmsd/_rmsd.pyx:212-213)."""
+    if txyz.dtype == np.float32 and txyz.flags.c_contiguous and txyz.shape[0]:
+        _center_inplace_atom_major(txyz)
+    r = rxyz[frame:frame + 1] if isinstance(rxyz, np.ndarray) else None
+    if r is not None and r.dtype == np.float32 and r.flags.c_contiguous:
+        _center_inplace_atom_major(r)
+
+
 def _i32_host(idx):
     return None if idx is None else np.ascontiguousarray(dev.indices_on_host(idx), dtype=np.int32)
 
@@ -143,10 +174,11 @@ def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
     txyz = _xyz_of(target)
     rxyz = _xyz_of(reference)
     idx, ridx, n_sel, frame = _validate(txyz, rxyz, frame, atom_indices, ref_atom_indices)
+    usable = False
     if superpose:
         # _rmsd.pyx:203-211: the cached traces are usable only with both
         # caches present and no selection; otherwise the reference warns.
-        usable = (getattr(reference, "_rmsd_traces", None) is not None
+        usable = (precentered and getattr(reference, "_rmsd_traces", None) is not None
                   and getattr(target, "_rmsd_traces", None) is not None
                   and atom_indices is None)
         if precentered and not usable:
@@ -156,8 +188,18 @@ def rmsd(target, reference, frame=0, atom_indices=None, ref_atom_indices=None,
         warnings.warn("in rmsd(), precentered is ignored when superpose=False", RuntimeWarning)
 
     same = _same_buffer(txyz, rxyz) and ref_atom_indices is None
+    centre_after = (_INPLACE and superpose and atom_indices is None and not usable
+                    and isinstance(txyz, np.ndarray))
+    if centre_after:
+        # the reference centres views of both arrays in place
+        for a in (txyz, rxyz):
+            if isinstance(a, np.ndarray) and not a.flags.writeable:
+                raise ValueError("buffer source array is read-only")
     if superpose and isinstance(txyz, np.ndarray) and not dev.is_device_array(rxyz):
-        return _rmsd_host(txyz, rxyz, frame, idx, ridx, n_sel, same)
+        out = _rmsd_host(txyz, rxyz, frame, idx, ridx, n_sel, same)
+        if centre_after:
+            _centre_like_reference(txyz, rxyz, frame)
+        return out
     lib = load()
     on_device = dev.is_device_array(txyz)
     T = dev.to_device_f32(txyz)

@@ -557,6 +557,258 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half
     }
 }
 
+// ------------------------------------------- split-MFMA kernel, half tiles
+// The same Gram and epilogue on 64 x 32 pair tiles: 4 waves (one per SIMD)
+// of 32 x 16 pairs each, and a 2-stage operand ring of 77,824 bytes, so that
+// two workgroups fit on a CU (2 waves per SIMD, at most 256 registers each).
+// The workgroups of a CU run independently, so one's FP64 QCP epilogue
+// overlaps the other's MFMA loop, where the 64 x 64 kernel (one workgroup
+// per CU) leaves the matrix cores idle during its epilogue.  Both halves of a
+// 64 x 64 tile come from consecutive blocks, so after the XCD remap they
+// share their 64 A rows in one L2.
+constexpr int kHBlock = 256;
+constexpr int kHRowsA = 64, kHRowsB = 32;
+constexpr int kHSlotsData = (kHRowsA + kHRowsB) * kSlots;   // 2,400 slots of 16 B
+constexpr int kHLoads = (kHSlotsData + kWave - 1) / kWave;    // 38 wave-instructions
+constexpr int kHStageBytes = kHLoads * kWave * 16;            // 38,912 B (512 B of slack)
+constexpr int kHStages = 2;
+constexpr size_t kHalfLds = (size_t)kHStages * kHStageBytes;  // 77,824
+constexpr int kHTileBBytes = kHRowsB * kSlots * 16;
+static_assert(kHLoads == 38, "load plan assumes 38 instructions per stage");
+static_assert(4 * 2 * 9 * 4 * kWave * 4 <= (int)kHalfLds, "epilogue park must fit the ring");
+
+// Waves 0-1 issue ten instructions of a stage, waves 2-3 nine (38 in all);
+// lanes of the last instruction past the data land in the stage's slack.
+__device__ __forceinline__ int hloads_first(int w) { return w < 2 ? 10 * w : 20 + 9 * (w - 2); }
+__device__ __forceinline__ int hloads_count(int w) { return w < 2 ? 10 : 9; }
+constexpr int kHMaxLoads = 10;
+
+template <typename Half>
+__device__ __forceinline__ void plan_loads_half(const GramArgs<Half>& g, long long i0,
+                                                long long j0, int n_chunks, int first, int lane,
+                                                const char* (&src)[kHMaxLoads]) {
+#pragma unroll
+    for (int k = 0; k < kHMaxLoads; ++k) {
+        int slot = (first + k) * kWave + lane;
+        if (slot >= kHSlotsData) slot = 0;         // slack lanes: a valid duplicate source
+        const int tile = slot >= kHRowsA * kSlots;  // 0: A rows, 1: B columns
+        const int sl = slot - tile * (kHRowsA * kSlots);
+        const int fr = sl / kSlots;
+        int r = sl - fr * kSlots;
+        if (r >= kSlots - 1) r = 0;                 // pad slot: harmless duplicate load
+        const long long nf = tile ? g.n_b : g.n_a;
+        long long f = (tile ? j0 : i0) + fr;
+        if (f >= nf) f = nf - 1;                    // rows past the end: never written out
+        const Half* base = tile ? g.split_b : g.split_a;
+        src[k] = reinterpret_cast<const char*>(base + (f * n_chunks * 6) * kChunk + r * 8);
+    }
+}
+
+template <typename Half>
+__device__ __forceinline__ void issue_stage_half(const char* const (&src)[kHMaxLoads], int first,
+                                                 int count, int c, char* stage) {
+    const long long adv = (long long)c * 6 * kChunk * (long long)sizeof(Half);
+#pragma unroll
+    for (int k = 0; k < kHMaxLoads; ++k) {
+        if (k < count)
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const void*>(src[k] + adv),
+                (__attribute__((address_space(3))) void*)(stage + (first + k) * kWave * 16), 16,
+                0, 0);
+    }
+}
+
+// Tile of block b: the 64 x 64 super-tile order of the full kernel, each
+// 64 x 64 tile split into two column halves taken by consecutive blocks.
+//   kRect false: upper triangle of A x A into the condensed vector;
+//   kRect true:  the A x B rectangle, tiles_b 32-column tiles per row block.
+template <typename Half, Split kSplit, bool kRect>
+__global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramArgs<Half> g) {
+    typedef typename Mfma<Half>::vec vec;
+    constexpr bool kExact = kSplit == Split::ExactHi;
+    constexpr int kSets = kExact ? 2 : 1;
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+
+    long long ti, tj;
+    const long long lin = xcd_remap(blockIdx.x, g.n_blocks);
+    long long i0, j0;
+    if constexpr (kRect) {
+        ti = lin / g.tiles_b;
+        tj = lin - ti * g.tiles_b;
+        i0 = ti * kHRowsA;
+        j0 = tj * kHRowsB;
+    } else {
+        const long long T = (g.n_a + kMTile - 1) / kMTile;
+        if (!super_tile(lin >> 1, T, ti, tj)) return;  // whole block
+        i0 = ti * kMTile;
+        j0 = tj * kMTile + kHRowsB * (lin & 1);
+        if (j0 >= g.n_a) return;                       // whole block: a half past the end
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int wi = w >> 1, wj = w & 1;
+    // a wave has work unless its 32 x 16 block lies on or below the diagonal
+    const bool active = kRect || j0 + 16 * wj + 15 > i0 + 32 * wi;
+    const int n_chunks = g.k_pad / kChunk;
+
+    f32x4 acc[kSets][9][2];
+#pragma unroll
+    for (int st = 0; st < kSets; ++st)
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) acc[st][e][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int quarter = lane >> 4;
+    int aoff[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+        aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
+    const int boff = kHRowsA * kSlots * 16 + (16 * wj + (lane & 15)) * kSlots * 16 + quarter * 16;
+
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int first = hloads_first(wu), count = hloads_count(wu);
+    const char* src[kHMaxLoads];
+    plan_loads_half<Half>(g, i0, j0, n_chunks, first, lane, src);
+    issue_stage_half<Half>(src, first, count, 0, lds);
+    for (int c = 0; c < n_chunks; ++c) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's chunk-c loads landed
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): its chunk c-1 LDS reads are done
+        __builtin_amdgcn_s_barrier();        // every wave: chunk c in LDS, stage c+1 free
+        if (c + 1 < n_chunks)
+            issue_stage_half<Half>(src, first, count, c + 1, lds + ((c + 1) & 1) * kHStageBytes);
+        if (!active) continue;
+        const char* st = lds + (c & 1) * kHStageBytes;
+        vec b[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) b[q] = *reinterpret_cast<const vec*>(st + boff + q * 64);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            vec a[2][2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+                    a[rb][p] = *reinterpret_cast<const vec*>(st + aoff[rb] + (2 * r + p) * 64);
+            constexpr int X = kSets - 1;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[0][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][0], b[2 * cc], acc[0][3 * r + cc][rb]);
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[X][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][0], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+                    acc[X][3 * r + cc][rb] =
+                        Mfma<Half>::run(a[rb][1], b[2 * cc], acc[X][3 * r + cc][rb]);
+            if constexpr (kExact) {
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb)
+                        acc[X][3 * r + cc][rb] =
+                            Mfma<Half>::run(a[rb][1], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
+            }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __syncthreads();
+
+    // Epilogue, two rounds (rb = 0, 1): every wave parks one 16-row block of
+    // its accumulators in LDS ([wave][set][entry][v][lane] floats, at most
+    // 4 x 2 x 9 x 4 x 64 x 4 B = 73,728 B), then the 256 threads solve the
+    // round's 1,024 pairs, four each, two at a time (qcp_msd2).
+    float* ep = reinterpret_cast<float*>(lds);
+    const double n = (double)g.n_sel;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        if (active) {
+#pragma unroll
+            for (int st = 0; st < kSets; ++st)
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        ep[(((w * kSets + st) * 9 + e) * 4 + v) * 64 + lane] = acc[st][e][rb][v];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int k = 0; k < 4; k += 2) {
+            double M2[2][9], ga[2], gb[2];
+            long long o2[2];
+            bool ok[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int p = threadIdx.x + kHBlock * (k + h);  // 0..1023
+                const int pw = p >> 8, v = (p >> 6) & 3, ln = p & 63;
+                const int pwi = pw >> 1, pwj = pw & 1;
+                const bool live = kRect || j0 + 16 * pwj + 15 > i0 + 32 * pwi;
+                const long long i = i0 + 32 * pwi + 16 * rb + 4 * (ln >> 4) + v;
+                const long long j = j0 + 16 * pwj + (ln & 15);
+                ok[h] = live && (kRect ? (i < g.n_a && j < g.n_b) : (i < j && j < g.n_a));
+                const long long ic = ok[h] ? i : 0, jc = ok[h] ? j : 0;
+                const double sij = (double)(g.scale_a[ic] * g.scale_b[jc]);
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    double val = (double)ep[(((pw * kSets + 0) * 9 + e) * 4 + v) * 64 + ln];
+                    if constexpr (kExact)
+                        val += (double)ep[(((pw * kSets + 1) * 9 + e) * 4 + v) * 64 + ln];
+                    M2[h][e] = val * sij;
+                }
+                ga[h] = g.traces_a[ic];
+                gb[h] = g.traces_b[jc];
+                o2[h] = ok[h] ? (kRect ? i * g.out_ld + j : condensed_index(i, j, g.n_a)) : 0;
+                if (!ok[h]) {
+                    // a pair not written out (its LDS slot may be stale): a
+                    // small well-conditioned stand-in (M = I, lambda = 3)
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) M2[h][e] = (e % 4 == 0) ? 1.0 : 0.0;
+                    ga[h] = 3.5;
+                    gb[h] = 3.5;
+                }
+            }
+            const bool any = ok[0] || ok[1];
+            if (!g.row_best && !any) continue;
+            double msd[2] = {0.0, 0.0};
+            if (any) qcp_msd2(M2[0], ga[0], gb[0], M2[1], ga[1], gb[1], n, msd[0], msd[1]);
+            if (!g.row_best) {
+                if (ok[0]) g.out[o2[0]] = sqrtf((float)msd[0]);
+                if (ok[1]) g.out[o2[1]] = sqrtf((float)msd[1]);
+                continue;
+            }
+            // nearest column of each row, as in the 64 x 64 kernel
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int p = threadIdx.x + kHBlock * (k + h);
+                const int pw = p >> 8, v = (p >> 6) & 3, ln = p & 63;
+                const long long i = i0 + 32 * (pw >> 1) + 16 * rb + 4 * (ln >> 4) + v;
+                const long long j = j0 + 16 * (pw & 1) + (ln & 15);
+                unsigned long long key =
+                    ok[h] ? ((unsigned long long)__float_as_uint(sqrtf((float)msd[h])) << 32) |
+                                (unsigned long long)(unsigned)j
+                          : ~0ull;
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1) {
+                    const unsigned hi = __shfl_xor((unsigned)(key >> 32), m, kWave);
+                    const unsigned lo = __shfl_xor((unsigned)key, m, kWave);
+                    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+                    key = o < key ? o : key;
+                }
+                if ((ln & 15) == 0 && key != ~0ull) atomicMin(g.row_best + i, key);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int k_pad_for(int n_sel) { return (n_sel + 63) / 64 * 64; }  // multiple of kChunk
@@ -633,6 +885,43 @@ int launch_gram(hipStream_t s, GramArgs<Half> g) {
     return launch_status("pairs_mfma_kernel");
 }
 
+template <typename Half, Split kSplit, bool kRect>
+int launch_gram_half(hipStream_t s, GramArgs<Half> g) {
+    static bool lds_attr_set = false;
+    if (!lds_attr_set) {
+        HR_CHECK_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void*>(pairs_mfma_half_kernel<Half, kSplit, kRect>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHalfLds));
+        lds_attr_set = true;
+    }
+    hipLaunchKernelGGL((pairs_mfma_half_kernel<Half, kSplit, kRect>), dim3((unsigned)g.n_blocks),
+                       dim3(kHBlock), kHalfLds, s, g);
+    return launch_status("pairs_mfma_half_kernel");
+}
+
+// HR_TUNE_PAIRS_TILE: 1 the 64 x 64 kernel, 2 (default) the 64 x 32 half-tile
+// kernel (20,000 x 500 f16exact on MI355X: 7.23 ms against 8.16 ms).
+bool use_half_tiles() { return tuning(HR_TUNE_PAIRS_TILE) != 1; }
+
+// Triangle or rectangle launch over prepared sets, in the chosen tile shape.
+template <typename Half, Split kSplit, bool kRect>
+int launch_pairs(hipStream_t s, GramArgs<Half> g) {
+    if (use_half_tiles()) {
+        if constexpr (kRect) {
+            g.tiles_b = (g.n_b + kHRowsB - 1) / kHRowsB;
+            g.n_blocks = (g.n_a + kHRowsA - 1) / kHRowsA * g.tiles_b;
+        } else {
+            g.n_blocks *= 2;
+        }
+        if (g.n_blocks >= (1LL << 31)) {
+            set_error("too many tiles (%lld)", g.n_blocks);
+            return HR_ERR_UNSUPPORTED;
+        }
+        return launch_gram_half<Half, kSplit, kRect>(s, g);
+    }
+    return launch_gram<Half, kSplit, kRect>(s, g);
+}
+
 template <typename Half, Split kSplit>
 int all_pairs_split(hipStream_t s, void* ws, const float* xyz, long long F, int n_atoms_total,
                     const int* idx, int n_sel, int K, float* out) {
@@ -654,7 +943,7 @@ int all_pairs_split(hipStream_t s, void* ws, const float* xyz, long long F, int 
         set_error("too many tiles (%lld)", g.n_blocks);
         return HR_ERR_UNSUPPORTED;
     }
-    return launch_gram<Half, kSplit, false>(s, g);
+    return launch_pairs<Half, kSplit, false>(s, g);
 }
 
 template <typename Half, Split kSplit>
@@ -681,7 +970,7 @@ int cross_split(hipStream_t s, const void* a, long long n_a, const void* b, long
         set_error("too many tiles (%lld)", g.n_blocks);
         return HR_ERR_UNSUPPORTED;
     }
-    return launch_gram<Half, kSplit, true>(s, g);
+    return launch_pairs<Half, kSplit, true>(s, g);
 }
 
 // Nearest column per row: keys start at all ones (a fill kernel, so the
@@ -732,7 +1021,7 @@ int cross_argmin(hipStream_t s, const void* a, long long n_a, const void* b, lon
     hipLaunchKernelGGL(fill_keys_kernel, dim3(blocks), dim3(256), 0, s, keys, n_a);
     int rc = launch_status("fill_keys_kernel");
     if (rc) return rc;
-    if ((rc = launch_gram<Half, kSplit, true>(s, g))) return rc;
+    if ((rc = launch_pairs<Half, kSplit, true>(s, g))) return rc;
     hipLaunchKernelGGL(unpack_keys_kernel, dim3(blocks), dim3(256), 0, s, keys, n_a, index,
                        value);
     return launch_status("unpack_keys_kernel");

@@ -113,18 +113,17 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
         const int n = s.n_atoms;
         run_group_batch(s.xyz, unit.base, unit.cnt, n, s.ref_sel, s.ref_sel + n,
                         s.ref_sel + 2 * n, s_res, lane, bx, by, bz, qx, qy, qz);
-        // the ring is empty: start the next unit's loads before this one's
-        // QCP solves, so their latency hides behind them
+        // start the next unit's loads before this one's QCP solves, so their
+        // latency hides behind them; unconditionally (past the table's end
+        // the last unit is loaded again and never used), since loads under a
+        // branch are waited for at its join
         const unsigned nu = n_waves + __builtin_amdgcn_readfirstlane(__shfl(next, 0, kWave));
-        DevUnit unit_n{};
-        DevSystem s_n{};
-        if ((long long)nu < n_units) {
-            unit_n = units[nu];
-            s_n = sys[unit_n.system];
-            prime_ring_batch(s_n.xyz, unit_n.base, unit_n.cnt, s_n.n_atoms, s_n.ref_sel,
-                             s_n.ref_sel + s_n.n_atoms, s_n.ref_sel + 2 * s_n.n_atoms, lane, bx,
-                             by, bz, qx, qy, qz);
-        }
+        const long long nuc = (long long)nu < n_units ? (long long)nu : n_units - 1;
+        const DevUnit unit_n = units[nuc];
+        const DevSystem s_n = sys[unit_n.system];
+        prime_ring_batch(s_n.xyz, unit_n.base, unit_n.cnt, s_n.n_atoms, s_n.ref_sel,
+                         s_n.ref_sel + s_n.n_atoms, s_n.ref_sel + 2 * s_n.n_atoms, lane, bx, by,
+                         bz, qx, qy, qz);
         const GlobalDouble* st = global_ptr(s.ref_stats);
         const double rs[4] = {st[0], st[1], st[2], st[3]};
         finish_group<Kind::Rmsd>(unit.base, unit.cnt, (double)n, rs, (long long)s.ref_frame, s.out,

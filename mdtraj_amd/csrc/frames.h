@@ -500,7 +500,9 @@ __device__ __forceinline__ void load_item_batch(const float* __restrict__ fr,
 }
 
 // Loads of the first kRingB - 1 items of a batched group into ring slots
-// 0 .. kRingB - 2.
+// 0 .. kRingB - 2.  Unconditional: an item past the group's end loads the
+// last item again (cached) and is never accumulated.  A load under a branch
+// would be merged at the join by a register copy, which waits for it at once.
 __device__ __forceinline__ void prime_ring_batch(
     const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
     const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
@@ -514,17 +516,21 @@ __device__ __forceinline__ void prime_ring_batch(
     const int n_items = (cnt + 3) / 4 * n_blk;
 #pragma unroll
     for (int r = 0; r < kRingB - 1; ++r) {
-        if (r < n_items) {
-            const int q = r / n_blk, b = r - q * n_blk;
-            const int k = 4 * q + row;
-            load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b, lane, rx,
-                            ry, rz, bx[r], by[r], bz[r], qx[r], qy[r], qz[r]);
-        }
+        const int rc = min(r, n_items - 1);
+        const int q = rc / n_blk, b = rc - q * n_blk;
+        const int k = 4 * q + row;
+        load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b, lane, rx,
+                        ry, rz, bx[r], by[r], bz[r], qx[r], qy[r], qz[r]);
     }
 }
 
 // The items of one batched group (ring primed by prime_ring_batch); on
-// return the group's row sums are parked in s_res and the ring is empty.
+// return the group's row sums are parked in s_res.  The item loop runs a
+// multiple of kRingB steps with no branch around a load: every step refills
+// the slot freed one step ago (an item past the end reloads the last one),
+// and steps past the last item accumulate zeros.  So each slot's loads stay
+// in flight for kRingB - 1 steps.  The ring holds loads of the group's last
+// item when it returns.
 __device__ __forceinline__ void run_group_batch(
     const float* __restrict__ xyz, long long base, int cnt, int n_atoms,
     const float* __restrict__ rx, const float* __restrict__ ry, const float* __restrict__ rz,
@@ -537,39 +543,40 @@ __device__ __forceinline__ void run_group_batch(
     const long long frame_stride = 3LL * n_atoms;
     const int n_blk = (n_atoms + kBlk - 1) / kBlk;
     const int n_items = (cnt + 3) / 4 * n_blk;
+    const int n_steps = (n_items + kRingB - 1) / kRingB * kRingB;
     Sums s = {};
-    for (int it0 = 0; it0 < n_items; it0 += kRingB) {
+    for (int it0 = 0; it0 < n_steps; it0 += kRingB) {
 #pragma unroll
         for (int r = 0; r < kRingB; ++r) {
             const int it = it0 + r;
-            if (it < n_items) {
-                const int nxt = it + kRingB - 1;
-                if (nxt < n_items) {
-                    const int q = nxt / n_blk, b = nxt - q * n_blk;
-                    const int k = 4 * q + row;
-                    const int slot = (r + kRingB - 1) % kRingB;
-                    load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b, lane,
-                                    rx, ry, rz, bx[slot], by[slot], bz[slot], qx[slot], qy[slot],
-                                    qz[slot]);
-                }
-                // atom 64 b + t + 16 u of the reference was loaded by lane
-                // 16 u + t: fetch it from there (cross-lane permute)
-                const int q = it / n_blk, b = it - q * n_blk;
-                const bool row_ok = 4 * q + row < cnt;
+            {
+                const int nxt = min(it + kRingB - 1, n_items - 1);
+                const int q = nxt / n_blk, b = nxt - q * n_blk;
+                const int k = 4 * q + row;
+                const int slot = (r + kRingB - 1) % kRingB;
+                load_item_batch(xyz + (base + (k < cnt ? k : 0)) * frame_stride, n_atoms, t, b,
+                                lane, rx, ry, rz, bx[slot], by[slot], bz[slot], qx[slot],
+                                qy[slot], qz[slot]);
+            }
+            // atom 64 b + t + 16 u of the reference was loaded by lane
+            // 16 u + t: fetch it from there (cross-lane permute)
+            const bool live = it < n_items;  // wave-uniform
+            const int itc = live ? it : n_items - 1;
+            const int q = itc / n_blk, b = itc - q * n_blk;
+            const bool row_ok = live && 4 * q + row < cnt;
 #pragma unroll
-                for (int u = 0; u < kItemAtoms; ++u) {
-                    const int src = u * kRowLanes + t;
-                    const bool ok = row_ok && b * kBlk + src < n_atoms;
-                    accumulate<Kind::Rmsd>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
-                                           valid_atom(ok, bz[r][u]), __shfl(qx[r], src, kWave),
-                                           __shfl(qy[r], src, kWave), __shfl(qz[r], src, kWave));
-                }
-                if (b == n_blk - 1) {
-                    row_reduce<Kind::Rmsd>(s);
-                    const int k = 4 * q + row;
-                    if (t == 0 && k < cnt) store_sums<Kind::Rmsd>(s_res + k * kNS, s);
-                    s = Sums{};
-                }
+            for (int u = 0; u < kItemAtoms; ++u) {
+                const int src = u * kRowLanes + t;
+                const bool ok = row_ok && b * kBlk + src < n_atoms;
+                accumulate<Kind::Rmsd>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                       valid_atom(ok, bz[r][u]), __shfl(qx[r], src, kWave),
+                                       __shfl(qy[r], src, kWave), __shfl(qz[r], src, kWave));
+            }
+            if (live && b == n_blk - 1) {
+                row_reduce<Kind::Rmsd>(s);
+                const int k = 4 * q + row;
+                if (t == 0 && k < cnt) store_sums<Kind::Rmsd>(s_res + k * kNS, s);
+                s = Sums{};
             }
         }
     }
