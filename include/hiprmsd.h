@@ -147,6 +147,20 @@ int hr_rmsd_one_vs_all_prepared(const float* target, int64_t n_frames, int32_t n
                                 const int32_t* atom_idx, int32_t n_sel, int64_t self_frame,
                                 float* out, const void* workspace, void* stream);
 
+/* hr_rmsd_one_vs_all with no allocation and no workspace argument.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The workspace lives in a static per-device array
+ * of 786,688 bytes, so selections are limited to
+ * hr_rmsd_one_vs_all_noalloc_max_sel() = 65,536 atoms (larger ones fail with
+ * HR_ERR_INVALID).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, or two threads) race. */
+size_t hr_rmsd_one_vs_all_noalloc_max_sel(void);
+int hr_rmsd_one_vs_all_noalloc(const float* target, int64_t n_frames, int32_t n_atoms_total,
+                               const int32_t* atom_idx, int32_t n_sel, const float* ref_frame,
+                               const int32_t* ref_idx, int64_t self_frame, float* out,
+                               void* stream);
+
 /* Same, for axis-major targets (n_frames, 3, rowstride) — replaces
  * msd_axis_major (rmsd/include/theobald_rmsd.h:13-14) as driven by
  * getMultipleRMSDs_axis_major (rmsd/_rmsd.pyx:500-557).  ref_frame_axis is

@@ -49,6 +49,9 @@ _SIGNATURES = {
     "hr_rmsd_one_vs_all": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
     "hr_rmsd_prepare_reference": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),
     "hr_rmsd_one_vs_all_prepared": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _i64, _vp, _vp, _vp]),
+    "hr_rmsd_one_vs_all_noalloc_max_sel": (_sz, []),
+    "hr_rmsd_one_vs_all_noalloc": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _i64, _vp,
+                                                  _vp]),
     "hr_rmsd_one_vs_all_axis_major": (ctypes.c_int, [_vp, _i64, _i32, _i32, _vp, _i32, _i64, _vp, _vp, _vp]),
     "hr_rmsd_nosuperpose": (ctypes.c_int, [_vp, _i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "hr_superpose_workspace_bytes": (_sz, [_i64, _i32]),
