@@ -545,7 +545,8 @@ def bench_batched(args):
                    "units": b.n_units, "frames_per_unit": best[0], "unit_kib": best[1], "waves_per_simd": best[2],
                    "graph_ms": ms},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile("batched_traffic.json", list(best))},
     }
 
 
