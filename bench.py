@@ -369,6 +369,28 @@ def time_all_pairs(X, mode_name, warmup, steps):
     return per, elapsed, out
 
 
+def settle_all_pairs(X, mode_name):
+    """The clock-settle phase of settle_clocks for the all-pairs kernel
+    (windows of 3 launches of ~7 ms)."""
+    import torch
+
+    from mdtraj_amd import _lib, _device as dev
+    from mdtraj_amd.allpairs import MODES
+
+    lib = _lib.load()
+    F, N = int(X.shape[0]), int(X.shape[1])
+    mode = MODES[mode_name]
+    out = dev.empty_f32(F * (F - 1) // 2)
+    nbytes = lib.hr_rmsd_all_pairs_workspace_bytes(F, N, mode)
+    ws = dev.workspace(nbytes)
+
+    def step():
+        _lib.call("hr_rmsd_all_pairs", X.data_ptr(), F, N, None, N, mode, out.data_ptr(),
+                  ws.data_ptr(), nbytes, dev.stream())
+
+    return settle_clocks(step, torch.cuda.current_stream(), min_launches=9, window=3, max_s=2.0)
+
+
 def all_pairs_figures(F, N, mode_name, ms):
     """Useful and executed TFLOP/s of one all-pairs launch of `ms` ms."""
     useful = 2.0 * (3 * F) ** 2 * N / 2   # upper triangle of the 3F x 3F Gram
@@ -418,6 +440,7 @@ def all_pairs_block(F=20_000, N=500, mode_name="f16exact"):
     X = gen.walk_device(seed=2, n_frames=F, n_atoms=N)
     torch.cuda.synchronize()
     n_pairs = F * (F - 1) // 2
+    settle = settle_all_pairs(X, mode_name)
     split_ms, _, out = time_all_pairs(X, mode_name, warmup=3, steps=5)
     del out
     fp32_ms, _, out = time_all_pairs(X, "fp32", warmup=1, steps=2)
@@ -444,6 +467,7 @@ def all_pairs_block(F=20_000, N=500, mode_name="f16exact"):
         "max_err_vs_fp64_nm": errs[mode_name],
         "fp32_max_err_vs_fp64_nm": errs["fp32"],
         "fp64_subset": f"seeded {n_sub}-frame subset, {n_sub_pairs} pairs, FP64 oracle {cpu_s:.1f} s",
+        "clock_settle": settle,
     }
 
 
@@ -458,6 +482,7 @@ def bench_all_pairs(args):
     X = gen.walk_device(seed=2, n_frames=F, n_atoms=N)
     torch.cuda.synchronize()
     n_pairs = F * (F - 1) // 2
+    settle = None if args.no_settle else settle_all_pairs(X, args.mode)
     per, elapsed, out = time_all_pairs(X, args.mode, args.warmup, args.steps)
     del out
     ms = float(np.mean(per))
@@ -475,6 +500,7 @@ def bench_all_pairs(args):
                   "bf16exact": "bf16x4-exact-hi"}[args.mode],
         "data": "synthetic (seeded random walk generated in HBM)",
         "config": {"workload": f"all-pairs RMSD matrix, {F} frames x {N} atoms, mode {args.mode}",
+                   "clock_settle": settle,
                    "kernel_ms": ms, "kernel_ms_per_launch": [float(v) for v in per],
                    "mfma_products": fig["mfma_products"],
                    "executed_tflops": fig["executed_tflops"],
@@ -1138,7 +1164,7 @@ def main():
                     choices=["fp32", "f16x3", "bf16x3", "f16exact", "bf16exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-settle", action="store_true",
-                    help="skip the clock-settle phase before warmup (one-vs-all)")
+                    help="skip the clock-settle phase before warmup (one-vs-all, all-pairs)")
     ap.add_argument("--no-all-pairs", action="store_true",
                     help="one-vs-all line without its all-pairs block")
     ap.add_argument("--batch-shape", default=None,
@@ -1152,9 +1178,9 @@ def main():
     ap.add_argument("--frames-grid", type=int, default=0, choices=[0, 1, 2],
                     help="grid of the per-frame kernel: 1 equal quad counts, 2 one wave "
                          "per resident slot (0 = library default, 2)")
-    ap.add_argument("--pairs-tile", type=int, default=0, choices=[0, 1, 2],
-                    help="split-precision Gram kernels: 1 64x64 tiles, 2 64x32 half tiles "
-                         "(0 = library default, 2)")
+    ap.add_argument("--pairs-tile", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
+                    help="split-precision Gram kernels: 1 64x64 tiles, 2-5 64x32 half tiles "
+                         "(see HR_TUNE_PAIRS_TILE; 0 = library default, 5)")
     ap.add_argument("--frames-row-lanes", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
                     help="lanes per frame of the per-frame kernel (0 = library default)")
     args = ap.parse_args()

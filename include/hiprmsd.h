@@ -85,9 +85,12 @@ int hr_device_cu_count(void);
  *   HR_TUNE_PAIRS_TILE             split-precision Gram kernels (all-pairs,
  *                                  cross, nearest column): 1 64 x 64 pair
  *                                  tiles, 8 waves, one workgroup per CU;
- *                                  2 (default) 64 x 32 tiles, 4 waves, two
- *                                  workgroups per CU.  Results do not depend
- *                                  on it. */
+ *                                  2-5 64 x 32 tiles, 4 waves, two
+ *                                  workgroups per CU: 25-slot LDS rows and
+ *                                  an epilogue through LDS (2), 26-slot rows
+ *                                  (3), the QCP solved from registers (4),
+ *                                  both (5, the default).  Results do not
+ *                                  depend on it. */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2

@@ -568,34 +568,45 @@ __global__ __launch_bounds__(kMBlock) void pairs_mfma_kernel(const GramArgs<Half
 // share their 64 A rows in one L2.
 constexpr int kHBlock = 256;
 constexpr int kHRowsA = 64, kHRowsB = 32;
-constexpr int kHSlotsData = (kHRowsA + kHRowsB) * kSlots;   // 2,400 slots of 16 B
-constexpr int kHLoads = (kHSlotsData + kWave - 1) / kWave;    // 38 wave-instructions
-constexpr int kHStageBytes = kHLoads * kWave * 16;            // 38,912 B (512 B of slack)
 constexpr int kHStages = 2;
-constexpr size_t kHalfLds = (size_t)kHStages * kHStageBytes;  // 77,824
-constexpr int kHTileBBytes = kHRowsB * kSlots * 16;
-static_assert(kHLoads == 38, "load plan assumes 38 instructions per stage");
-static_assert(4 * 2 * 9 * 4 * kWave * 4 <= (int)kHalfLds, "epilogue park must fit the ring");
 
-// Waves 0-1 issue ten instructions of a stage, waves 2-3 nine (38 in all);
-// lanes of the last instruction past the data land in the stage's slack.
-__device__ __forceinline__ int hloads_first(int w) { return w < 2 ? 10 * w : 20 + 9 * (w - 2); }
-__device__ __forceinline__ int hloads_count(int w) { return w < 2 ? 10 : 9; }
+// Operand ring of the half-tile kernel for frame rows of kS 16-byte slots
+// (24 data + padding): data slots, load instructions per stage (the last
+// one's lanes past the data land in the stage's slack), bytes.
+template <int kS>
+struct HalfRing {
+    static constexpr int kSlotsData = (kHRowsA + kHRowsB) * kS;
+    static constexpr int kLoads = (kSlotsData + kWave - 1) / kWave;
+    static constexpr int kStageBytes = kLoads * kWave * 16;
+    static constexpr size_t kLds = (size_t)kHStages * kStageBytes;
+    static_assert(kLds <= 81920, "two workgroups per CU must fit 160 KiB of LDS");
+    static_assert(4 * 2 * 9 * 4 * kWave * 4 <= (int)kLds, "epilogue park must fit the ring");
+    // waves 0 .. kLoads % 4 - 1 issue one instruction more than the others
+    static __device__ __forceinline__ int first(int w) {
+        const int base = kLoads / 4, extra = kLoads % 4;
+        return w * base + (w < extra ? w : extra);
+    }
+    static __device__ __forceinline__ int count(int w) {
+        return kLoads / 4 + (w < kLoads % 4 ? 1 : 0);
+    }
+};
 constexpr int kHMaxLoads = 10;
+static_assert(HalfRing<25>::kLoads == 38 && HalfRing<26>::kLoads == 39, "load plans");
+constexpr size_t kHalfLds = HalfRing<26>::kLds;  // dynamic LDS of either ring: 79,872 B
 
-template <typename Half>
+template <typename Half, int kS>
 __device__ __forceinline__ void plan_loads_half(const GramArgs<Half>& g, long long i0,
                                                 long long j0, int n_chunks, int first, int lane,
                                                 const char* (&src)[kHMaxLoads]) {
 #pragma unroll
     for (int k = 0; k < kHMaxLoads; ++k) {
         int slot = (first + k) * kWave + lane;
-        if (slot >= kHSlotsData) slot = 0;         // slack lanes: a valid duplicate source
-        const int tile = slot >= kHRowsA * kSlots;  // 0: A rows, 1: B columns
-        const int sl = slot - tile * (kHRowsA * kSlots);
-        const int fr = sl / kSlots;
-        int r = sl - fr * kSlots;
-        if (r >= kSlots - 1) r = 0;                 // pad slot: harmless duplicate load
+        if (slot >= HalfRing<kS>::kSlotsData) slot = 0;  // slack lanes: a valid duplicate source
+        const int tile = slot >= kHRowsA * kS;           // 0: A rows, 1: B columns
+        const int sl = slot - tile * (kHRowsA * kS);
+        const int fr = sl / kS;
+        int r = sl - fr * kS;
+        if (r >= 24) r = 0;                              // pad slots: harmless duplicate loads
         const long long nf = tile ? g.n_b : g.n_a;
         long long f = (tile ? j0 : i0) + fr;
         if (f >= nf) f = nf - 1;                    // rows past the end: never written out
@@ -622,7 +633,12 @@ __device__ __forceinline__ void issue_stage_half(const char* const (&src)[kHMaxL
 // 64 x 64 tile split into two column halves taken by consecutive blocks.
 //   kRect false: upper triangle of A x A into the condensed vector;
 //   kRect true:  the A x B rectangle, tiles_b 32-column tiles per row block.
-template <typename Half, Split kSplit, bool kRect>
+//   kS: 16-byte slots per frame row in LDS (25 or 26; which one reads the
+//       operands without bank conflicts depends on the LDS's bank count).
+//   kRegEpi: every lane solves the QCP of the 8 pairs its accumulators hold,
+//       straight from registers; otherwise the accumulators are parked in LDS
+//       and all 256 threads share the solves.
+template <typename Half, Split kSplit, bool kRect, int kS, bool kRegEpi>
 __global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramArgs<Half> g) {
     typedef typename Mfma<Half>::vec vec;
     constexpr bool kExact = kSplit == Split::ExactHi;
@@ -660,25 +676,26 @@ __global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramA
             for (int rb = 0; rb < 2; ++rb) acc[st][e][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int quarter = lane >> 4;
+    constexpr int kStageBytes = HalfRing<kS>::kStageBytes;
     int aoff[2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
-        aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kSlots * 16 + quarter * 16;
-    const int boff = kHRowsA * kSlots * 16 + (16 * wj + (lane & 15)) * kSlots * 16 + quarter * 16;
+        aoff[rb] = (32 * wi + 16 * rb + (lane & 15)) * kS * 16 + quarter * 16;
+    const int boff = kHRowsA * kS * 16 + (16 * wj + (lane & 15)) * kS * 16 + quarter * 16;
 
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    const int first = hloads_first(wu), count = hloads_count(wu);
+    const int first = HalfRing<kS>::first(wu), count = HalfRing<kS>::count(wu);
     const char* src[kHMaxLoads];
-    plan_loads_half<Half>(g, i0, j0, n_chunks, first, lane, src);
+    plan_loads_half<Half, kS>(g, i0, j0, n_chunks, first, lane, src);
     issue_stage_half<Half>(src, first, count, 0, lds);
     for (int c = 0; c < n_chunks; ++c) {
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's chunk-c loads landed
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): its chunk c-1 LDS reads are done
         __builtin_amdgcn_s_barrier();        // every wave: chunk c in LDS, stage c+1 free
         if (c + 1 < n_chunks)
-            issue_stage_half<Half>(src, first, count, c + 1, lds + ((c + 1) & 1) * kHStageBytes);
+            issue_stage_half<Half>(src, first, count, c + 1, lds + ((c + 1) & 1) * kStageBytes);
         if (!active) continue;
-        const char* st = lds + (c & 1) * kHStageBytes;
+        const char* st = lds + (c & 1) * kStageBytes;
         vec b[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) b[q] = *reinterpret_cast<const vec*>(st + boff + q * 64);
@@ -718,6 +735,77 @@ __global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramA
                             Mfma<Half>::run(a[rb][1], b[2 * cc + 1], acc[X][3 * r + cc][rb]);
             }
         }
+    }
+    if constexpr (kRegEpi) {
+        // Lane l holds, for rb and v, the nine entries of pair (row
+        // 32 wi + 16 rb + 4 (l / 16) + v, column 16 wj + l % 16): its QCP runs
+        // from registers, two pairs at a time (qcp_msd2), no LDS and no
+        // barrier.  A wave with no pair in the triangle has nothing to do.
+        if (!active && !g.row_best) return;
+        const double nn = (double)g.n_sel;
+        const long long j = j0 + 16 * wj + (lane & 15);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+            for (int v0 = 0; v0 < 4; v0 += 2) {
+                double M2[2][9], ga[2], gb[2];
+                long long o2[2];
+                bool ok[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int v = v0 + h;
+                    const long long i = i0 + 32 * wi + 16 * rb + 4 * (lane >> 4) + v;
+                    ok[h] = active &&
+                            (kRect ? (i < g.n_a && j < g.n_b) : (i < j && j < g.n_a));
+                    const long long ic = ok[h] ? i : 0, jc = ok[h] ? j : 0;
+                    const double sij = (double)(g.scale_a[ic] * g.scale_b[jc]);
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) {
+                        double val = (double)acc[0][e][rb][v];
+                        if constexpr (kExact) val += (double)acc[1][e][rb][v];
+                        M2[h][e] = val * sij;
+                    }
+                    ga[h] = g.traces_a[ic];
+                    gb[h] = g.traces_b[jc];
+                    o2[h] = ok[h] ? (kRect ? i * g.out_ld + j : condensed_index(i, j, g.n_a)) : 0;
+                    if (!ok[h]) {
+                        // a pair not written out: a small well-conditioned
+                        // stand-in (M = I, lambda = 3)
+#pragma unroll
+                        for (int e = 0; e < 9; ++e) M2[h][e] = (e % 4 == 0) ? 1.0 : 0.0;
+                        ga[h] = 3.5;
+                        gb[h] = 3.5;
+                    }
+                }
+                const bool any = ok[0] || ok[1];
+                double msd[2] = {0.0, 0.0};
+                if (any) qcp_msd2(M2[0], ga[0], gb[0], M2[1], ga[1], gb[1], nn, msd[0], msd[1]);
+                if (!g.row_best) {
+                    if (ok[0]) g.out[o2[0]] = sqrtf((float)msd[0]);
+                    if (ok[1]) g.out[o2[1]] = sqrtf((float)msd[1]);
+                    continue;
+                }
+                // nearest column of each row: the 16 lanes of a lane group
+                // share the row and hold 16 consecutive columns
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const long long i = i0 + 32 * wi + 16 * rb + 4 * (lane >> 4) + v0 + h;
+                    unsigned long long key =
+                        ok[h] ? ((unsigned long long)__float_as_uint(sqrtf((float)msd[h])) << 32) |
+                                    (unsigned long long)(unsigned)j
+                              : ~0ull;
+#pragma unroll
+                    for (int m = 1; m < 16; m <<= 1) {
+                        const unsigned hi = __shfl_xor((unsigned)(key >> 32), m, kWave);
+                        const unsigned lo = __shfl_xor((unsigned)key, m, kWave);
+                        const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+                        key = o < key ? o : key;
+                    }
+                    if ((lane & 15) == 0 && key != ~0ull) atomicMin(g.row_best + i, key);
+                }
+            }
+        }
+        return;
     }
     __builtin_amdgcn_s_waitcnt(0xC07F);
     __syncthreads();
@@ -885,22 +973,35 @@ int launch_gram(hipStream_t s, GramArgs<Half> g) {
     return launch_status("pairs_mfma_kernel");
 }
 
-template <typename Half, Split kSplit, bool kRect>
-int launch_gram_half(hipStream_t s, GramArgs<Half> g) {
+template <typename Half, Split kSplit, bool kRect, int kS, bool kRegEpi>
+int launch_gram_half_v(hipStream_t s, const GramArgs<Half>& g) {
+    auto k = pairs_mfma_half_kernel<Half, kSplit, kRect, kS, kRegEpi>;
     static bool lds_attr_set = false;
     if (!lds_attr_set) {
-        HR_CHECK_HIP(hipFuncSetAttribute(
-            reinterpret_cast<const void*>(pairs_mfma_half_kernel<Half, kSplit, kRect>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kHalfLds));
+        HR_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)HalfRing<kS>::kLds));
         lds_attr_set = true;
     }
-    hipLaunchKernelGGL((pairs_mfma_half_kernel<Half, kSplit, kRect>), dim3((unsigned)g.n_blocks),
-                       dim3(kHBlock), kHalfLds, s, g);
+    hipLaunchKernelGGL(k, dim3((unsigned)g.n_blocks), dim3(kHBlock), HalfRing<kS>::kLds, s, g);
     return launch_status("pairs_mfma_half_kernel");
 }
 
-// HR_TUNE_PAIRS_TILE: 1 the 64 x 64 kernel, 2 (default) the 64 x 32 half-tile
-// kernel (20,000 x 500 f16exact on MI355X: 7.23 ms against 8.16 ms).
+// HR_TUNE_PAIRS_TILE: 1 the 64 x 64 kernel; 2-5 the 64 x 32 half-tile kernel
+// with 25-slot rows and the LDS epilogue (2), 26-slot rows (3), the register
+// epilogue (4), both (5, the default).  20,000 x 500 f16exact on MI355X, ms
+// per matrix: 8.16 (1), 7.18 (2), 7.05-7.13 (3), 6.85 (4), 6.73-6.78 (5);
+// every variant writes the same bits.
+template <typename Half, Split kSplit, bool kRect>
+int launch_gram_half(hipStream_t s, const GramArgs<Half>& g) {
+    switch (tuning(HR_TUNE_PAIRS_TILE)) {
+        case 2: return launch_gram_half_v<Half, kSplit, kRect, 25, false>(s, g);
+        case 3: return launch_gram_half_v<Half, kSplit, kRect, 26, false>(s, g);
+        case 4: return launch_gram_half_v<Half, kSplit, kRect, 25, true>(s, g);
+        default: return launch_gram_half_v<Half, kSplit, kRect, 26, true>(s, g);
+    }
+}
+
 bool use_half_tiles() { return tuning(HR_TUNE_PAIRS_TILE) != 1; }
 
 // Triangle or rectangle launch over prepared sets, in the chosen tile shape.

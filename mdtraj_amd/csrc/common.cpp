@@ -89,8 +89,8 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0, 1 or 2 (got %d)", value);
         return HR_ERR_INVALID;
     }
-    if (knob == HR_TUNE_PAIRS_TILE && value != 0 && value != 1 && value != 2) {
-        hr::set_error("HR_TUNE_PAIRS_TILE must be 0, 1 or 2 (got %d)", value);
+    if (knob == HR_TUNE_PAIRS_TILE && (value < 0 || value > 5)) {
+        hr::set_error("HR_TUNE_PAIRS_TILE must be 0 .. 5 (got %d)", value);
         return HR_ERR_INVALID;
     }
     if (knob == HR_TUNE_HOST_DIRECT_KIB && value < -1) {

@@ -145,9 +145,11 @@ __host__ __device__ __forceinline__ double recip(double d) {
 // (near-)double roots -- the two largest eigenvalues of K (nearly) coincide,
 // as for a centred two-atom (collinear) structure -- where P's values
 // determine the root only to about sqrt(eps) lambda.
-// Coefficients of P(x) = x^4 + c2 x^2 + c1 x + c0 and the Frobenius bound.
+// Coefficients of P(x) = x^4 + c2 x^2 + c1 x + c0 and |M|_F^2, whose root
+// gives the Frobenius bound (taken only when the start e0 fails its check,
+// so the square root is not paid on the common path).
 struct Quartic {
-    double c2, c1, c0, frob;
+    double c2, c1, c0, mm;
 };
 
 // Newton state: the last point verified above every root, P, P', P'' there;
@@ -173,7 +175,7 @@ __host__ __device__ __forceinline__ NewtonState newton_start(const Quartic& q, d
     quartic_eval(q, s.x, s.p, s.d1, s.d2);
     s.run = s.p > 0.0 && s.d1 > 0.0 && s.d2 > 0.0 && s.x > 0.0;
     if (!s.run) {
-        s.x = 2.0 * q.frob * (1.0 + 1e-12) + 1e-300;
+        s.x = 2.0 * sqrt(q.mm) * (1.0 + 1e-12) + 1e-300;
         quartic_eval(q, s.x, s.p, s.d1, s.d2);
         s.run = s.p > 0.0 && s.d1 > 0.0 && s.d2 > 0.0 && s.x > 0.0;
     }
@@ -213,13 +215,17 @@ __host__ __device__ __forceinline__ double newton_finish(const Quartic& q, const
     return x - s.p * recip(s.d1);
 }
 
-__host__ __device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
-                                                   double m_frob, bool* resolved) {
-    const Quartic q{c2, c1, c0, m_frob};
+__host__ __device__ inline double qcp_largest_root(const Quartic& q, double e0, bool* resolved) {
     NewtonState s = newton_start(q, e0);
     const bool started = s.run;
     for (int it = 0; it < 64 && s.run; ++it) newton_step(q, s);
     return newton_finish(q, s, started, resolved);
+}
+
+// The same from the coefficients and |M|_F.
+__host__ __device__ inline double qcp_largest_root(double c2, double c1, double c0, double e0,
+                                                   double m_frob, bool* resolved) {
+    return qcp_largest_root(Quartic{c2, c1, c0, m_frob * m_frob}, e0, resolved);
 }
 
 // Key-matrix entries of Horn/Theobald for M[r*3+c] = sum a_r b_c, in the
@@ -390,7 +396,7 @@ __host__ __device__ __forceinline__ Quartic qcp_quartic(const double M[9], const
                       k.k01 * (k.k01 * a - k.k12 * d + k.k13 * e) +
                       k.k02 * (k.k01 * b - k.k11 * d + k.k13 * f) -
                       k.k03 * (k.k01 * c - k.k11 * e + k.k12 * f);
-    return Quartic{-2.0 * mm, -8.0 * det_m, c0, sqrt(mm)};
+    return Quartic{-2.0 * mm, -8.0 * det_m, c0, mm};
 }
 
 template <bool kRot>
@@ -398,10 +404,10 @@ __host__ __device__ inline double qcp_solve(const double M[9], double e0, double
     const KeyMatrix k = key_matrix(M);
     const Quartic q = qcp_quartic(M, k);
     bool resolved = true;
-    double lam = qcp_largest_root(q.c2, q.c1, q.c0, e0, q.frob, &resolved);
+    double lam = qcp_largest_root(q, e0, &resolved);
     if constexpr (kRot) {
         if (resolved && qcp_rotation(k, lam, R)) return lam;
-        if (!(q.frob > 0.0)) {
+        if (!(q.mm > 0.0)) {
             quat_rotation(1.0, 0.0, 0.0, 0.0, R);
             return 0.0;
         }

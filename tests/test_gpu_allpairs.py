@@ -68,17 +68,18 @@ def test_cross_batch_independence(gpu):
 
 @pytest.mark.parametrize("mode", ["f16exact", "f16x3", "bf16exact", "bf16x3"])
 def test_half_tiles_equal_full_tiles(gpu, mode):
-    """The 64 x 32 half-tile kernel (two workgroups per CU) runs the same
-    MFMA sequence per pair block and the same epilogue as the 64 x 64 kernel:
-    the triangle, the rectangle and the nearest column agree bit for bit,
-    over sizes that leave ragged tiles and a 32-column half past the end."""
+    """The 64 x 32 half-tile kernel (two workgroups per CU) and its variants
+    (26-slot rows, register epilogue) run the same MFMA sequence per pair
+    block and the same QCP as the 64 x 64 kernel: the triangle, the
+    rectangle and the nearest column agree bit for bit, over sizes that leave
+    ragged tiles and a 32-column half past the end."""
     from mdtraj_amd import _lib
     from mdtraj_amd.allpairs import SplitSet, nearest, rmsd_cross
 
     x = gen.walk(seed=61, n_frames=333, n_atoms=150)
     got = {}
     try:
-        for tile in (1, 2):
+        for tile in (1, 2, 3, 4, 5):
             _lib.call("hr_set_tuning", _lib.HR_TUNE_PAIRS_TILE, tile)
             tri = md.rmsd_matrix(x, mode=mode)
             a, b = SplitSet(x[:97], mode=mode), SplitSet(x[97:], mode=mode)
@@ -87,7 +88,8 @@ def test_half_tiles_equal_full_tiles(gpu, mode):
             got[tile] = (tri, rect, dev.to_host(idx), dev.to_host(val))
     finally:
         _lib.call("hr_set_tuning", _lib.HR_TUNE_PAIRS_TILE, 0)
-    for u, v in zip(got[1], got[2]):
-        assert np.array_equal(u, v)
+    for tile in (2, 3, 4, 5):
+        for u, v in zip(got[1], got[tile]):
+            assert np.array_equal(u, v), tile
     want = fp64.condensed(fp64.rmsd_matrix(x))
     assert np.abs(got[2][0] - want).max() < TOL[mode]
