@@ -199,13 +199,81 @@ struct BitReader {
     }
 };
 
-__device__ __forceinline__ void put_atom(float* __restrict__ dst, const int32_t (&c)[3],
-                                         float inv) {
+// A lane's decoded floats go out through a ring of two 128-byte lines in
+// LDS.  Once per decoded run the lines that are complete are written: a line
+// wholly inside the frame's output as eight 16-byte stores, so HBM sees full
+// lines; the two lines a frame shares with its neighbours float by float.
+// Written directly, the 64 lanes of a wave each dribbled 4-byte stores into 64
+// different lines, which were evicted part-written and merged by
+// read-modify-write: 2.3 times the algorithmic traffic.  Positions are
+// absolute float addresses (byte address / 4), so ring lines are 128-byte
+// lines of memory whatever the array's offset.
+//
+// Capacity: after a flush at most 31 floats of the current line are pending
+// and one run adds at most 11 atoms (a 5-bit run length of at most 30 small
+// atoms in threes, plus the large atom), 33 floats: 64 floats, the ring.
+constexpr int kLineFloats = 32;
+constexpr int kRingFloats = 2 * kLineFloats;
+constexpr int kRingStride = kRingFloats + 4;  // 16-byte aligned rows, spread over the banks
+constexpr int kMaxFloatsPerRun = 3 * (1 + 30 / 3);
+static_assert(kLineFloats - 1 + kMaxFloatsPerRun <= kRingFloats, "ring too small for one run");
+
+struct LineWriter {
+    float* __restrict__ ring;  // this lane's ring in LDS
+    uint64_t pos;              // absolute float address of the next float
+    uint64_t line;             // first line (absolute float address) not yet written
+    uint64_t beg, end;         // the frame's floats
+
+    __device__ __forceinline__ void start(float* r, uint64_t first, uint64_t n) {
+        ring = r;
+        pos = beg = first;
+        end = first + n;
+        line = first & ~(uint64_t)(kLineFloats - 1);
+    }
+
+    __device__ __forceinline__ void put(float v) {
+        ring[pos & (kRingFloats - 1)] = v;
+        ++pos;
+    }
+
     // the format's rounding: the integer to float, times float(1 / precision)
-    dst[0] = __fmul_rn((float)c[0], inv);
-    dst[1] = __fmul_rn((float)c[1], inv);
-    dst[2] = __fmul_rn((float)c[2], inv);
-}
+    __device__ __forceinline__ void put_atom(const int32_t (&c)[3], float inv) {
+        put(__fmul_rn((float)c[0], inv));
+        put(__fmul_rn((float)c[1], inv));
+        put(__fmul_rn((float)c[2], inv));
+    }
+
+    // writes the line at `line` holding the floats below `upto`
+    __device__ __forceinline__ void write_line(uint64_t upto) {
+        const float* src = ring + (line & (kRingFloats - 1));
+        if (line >= beg && line + kLineFloats <= upto) {
+            typedef float v4f __attribute__((ext_vector_type(4)));
+            const v4f* s4 = reinterpret_cast<const v4f*>(src);
+            auto* dst = global_out(reinterpret_cast<v4f*>(line << 2));
+#pragma unroll
+            for (int k = 0; k < kLineFloats / 4; ++k) dst[k] = s4[k];
+        } else {
+            for (int k = 0; k < kLineFloats; ++k) {
+                const uint64_t p = line + k;
+                if (p >= beg && p < upto) *global_out(reinterpret_cast<float*>(p << 2)) = src[k];
+            }
+        }
+    }
+
+    // writes every complete line
+    __device__ __forceinline__ void flush() {
+        while (line + kLineFloats <= pos) {
+            write_line(pos);
+            line += kLineFloats;
+        }
+    }
+
+    // writes what is left: the frame's last, partial line
+    __device__ __forceinline__ void finish() {
+        flush();
+        if (line < pos) write_line(pos);
+    }
+};
 
 // One frame per lane.  offsets[f] .. offsets[f + 1] (or n_bytes) bound the
 // frame's bytes; the host has checked magic and sizes when it built them.
@@ -214,6 +282,7 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
     const long long* __restrict__ offsets, long long n_frames, int n_atoms,
     float* __restrict__ xyz, float* __restrict__ time, int* __restrict__ step,
     float* __restrict__ box, float* __restrict__ prec, int* __restrict__ status) {
+    __shared__ __align__(16) float s_ring[kWave * kRingStride];
     const long long f = (long long)blockIdx.x * kWave + threadIdx.x;
     if (f >= n_frames) return;
     const long long beg = offsets[f];
@@ -298,6 +367,9 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
     const double inv_size2 = 1.0 / (double)sizeint[2], inv_size1 = 1.0 / (double)sizeint[1];
     BitReader br;
     br.start(pw + 23, (int)((nbytes + 3) / 4));
+    const uint64_t first = (reinterpret_cast<uintptr_t>(out) >> 2);
+    LineWriter wr;
+    wr.start(s_ring + threadIdx.x * kRingStride, first, 3ull * (uint64_t)n_atoms);
     int n_out = 0;
     int run = 0;  // a clear flag bit keeps the previous run length
     while (n_out < n_atoms) {
@@ -324,7 +396,8 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
             break;
         }
         if (run == 0) {
-            put_atom(out + 3 * n_out++, big, inv);
+            wr.put_atom(big, inv);
+            ++n_out;
         } else {
             // deltas from the atom before; the first small atom precedes the
             // large one in the output (stored swapped, for water)
@@ -334,14 +407,19 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
                 br.ints3(smallidx, sizesmall, sizesmall, sizesmall, 0.0, 0.0, d);
                 for (int c = 0; c < 3; ++c)
                     prev[c] = (int32_t)((uint32_t)d[c] + (uint32_t)prev[c] - (uint32_t)smallnum);
-                put_atom(out + 3 * n_out++, prev, inv);
-                if (k == 0) put_atom(out + 3 * n_out++, big, inv);
+                wr.put_atom(prev, inv);
+                ++n_out;
+                if (k == 0) {
+                    wr.put_atom(big, inv);
+                    ++n_out;
+                }
             }
             if (br.bad) {
                 st = HR_XTC_STREAM;
                 break;
             }
         }
+        wr.flush();
         smallidx += is_smaller;
         if (smallidx < kFirstIdx || smallidx >= kLastIdx) {
             st = HR_XTC_STREAM;
@@ -356,6 +434,7 @@ __global__ __launch_bounds__(kWave) void xtc_decode_kernel(
         }
         sizesmall = (uint32_t)kMagicInts[smallidx];
     }
+    wr.finish();
     status[f] = st;
 }
 
@@ -429,6 +508,7 @@ int hr_xtc_decode(const void* data, int64_t n_bytes, const int64_t* offsets, int
     HR_REQUIRE(data && offsets && xyz && time && step && box && prec && status,
                "NULL pointer argument");
     HR_REQUIRE((reinterpret_cast<uintptr_t>(data) & 3) == 0, "data must be 4-byte aligned");
+    HR_REQUIRE((reinterpret_cast<uintptr_t>(xyz) & 3) == 0, "xyz must be 4-byte aligned");
     const long long blocks = (n_frames + kWave - 1) / kWave;
     hipLaunchKernelGGL(xtc_decode_kernel, dim3((unsigned)blocks), dim3(kWave), 0,
                        (hipStream_t)stream, static_cast<const unsigned char*>(data),
