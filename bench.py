@@ -539,12 +539,16 @@ def bench_batched(args):
     torch.cuda.synchronize()
     if args.batch_shape:
         best = tuple(int(v) for v in args.batch_shape.split(","))
-        log(f"shape given: fpu={best[0]} kib={best[1]} waves={best[2]} (autotuner skipped)")
+        if len(best) == 3:
+            best = best + (1,)  # a unit shape: the ticket schedule
+        log(f"shape given: fpu={best[0]} kib={best[1]} waves={best[2]} schedule={best[3]} "
+            "(autotuner skipped)")
     else:
         best, table = autotune.tune_batch(xs, reps=3)
-        log("autotuner: " + ", ".join(f"fpu={k[0]} kib={k[1]} waves={k[2]}: {v:.3f} ms"
-                                      for k, v in table.items()))
-    b = BatchRMSD(xs, frames_per_unit=best[0], unit_kib=best[1], waves=best[2])
+        log("autotuner: " + ", ".join(f"fpu={k[0]} kib={k[1]} waves={k[2]} schedule={k[3]}: "
+                                      f"{v:.3f} ms" for k, v in table.items()))
+    b = BatchRMSD(xs, frames_per_unit=best[0], unit_kib=best[1], waves=best[2],
+                  schedule=best[3])
     for _ in range(args.warmup):
         b.run()
     torch.cuda.synchronize()
@@ -568,7 +572,7 @@ def bench_batched(args):
         "config": {"workload": "512 systems, atoms log-uniform [50, 5000], frames log-uniform "
                                "[10, 2000], sigma 0.03 nm, vs own frame 0",
                    "total_frames": b.total_frames, "total_bytes": b.total_bytes,
-                   "units": b.n_units, "frames_per_unit": best[0], "unit_kib": best[1], "waves_per_simd": best[2],
+                   "units": b.n_units, "frames_per_unit": best[0], "unit_kib": best[1], "waves_per_simd": best[2], "schedule": best[3],
                    "graph_ms": ms},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS,

@@ -62,12 +62,18 @@ int hr_device_cu_count(void);
  *                                  or 4, 8, 16, 32, 64 (16 down to 1 frames
  *                                  per wave)
  *   HR_TUNE_BATCH_SCHEDULE         work split of batch plans created after
- *                                  the call: 1 (default) work units largest
- *                                  first, each wave's first unit its wave
- *                                  index and the rest from an atomic ticket;
- *                                  2 a static split of all systems' frames
- *                                  into one contiguous, cost-balanced range
- *                                  per wave (no tile shape to tune)
+ *                                  the call: 3 (default) a static split of
+ *                                  all systems' frames into one contiguous,
+ *                                  cost-balanced range per workgroup, each
+ *                                  system's reference staged in LDS (up to
+ *                                  ~4,400 atoms; the rest of a larger one
+ *                                  read from global memory) and the frame
+ *                                  sums kept in registers; 1 work units
+ *                                  largest first, each wave's first unit its
+ *                                  wave index and the rest from an atomic
+ *                                  ticket (the tile shape applies to this
+ *                                  one only); 2 the static split per wave,
+ *                                  the reference read from global memory
  *   HR_TUNE_SUPERPOSE_CHUNK_MIB    hr_superpose fits and rotates the frames
  *                                  in chunks of about this many MiB of
  *                                  coordinates (a fit pass, then a rotation

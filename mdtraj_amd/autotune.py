@@ -1,16 +1,18 @@
-"""Autotuner for the batched plan's tile shape.
+"""Autotuner for the batched plan's work split and tile shape.
 
 Candidates are (frames per work unit, KiB cap per unit, waves per SIMD of
-the scoring kernel).  Every unit is scored by one wave, whose bandwidth is
-bounded by the loads it keeps in flight, so the batch ends with the largest
-units still running: small units (and a byte cap, which shortens the units
-of the largest systems) shorten that tail, large units amortise the per-unit
-reference reads and QCP; 4 waves per SIMD keeps more loads in flight at the
-cost of a few spilled registers.  Each candidate's plan is built and
-timed on the actual batch (graph launches, HIP events); the winner is cached
-per batch signature (system count and log2 bins of frame and atom counts) in
-memory and in a JSON file (``$MDTRAJ_AMD_TUNE_CACHE``, default
-``~/.cache/mdtraj_amd/autotune.json``).
+the scoring kernel, schedule).  Schedule 3 (one cost-balanced range per
+workgroup, the reference in LDS) has no unit shape.  Under schedule 1
+(work units from a ticket) every unit is scored by one wave, whose
+bandwidth is bounded by the loads it keeps in flight, so the batch ends with
+the largest units still running: small units (and a byte cap, which
+shortens the units of the largest systems) shorten that tail, large units
+amortise the per-unit reference reads and QCP; 4 waves per SIMD keeps more
+loads in flight at the cost of a few spilled registers.  Each candidate's
+plan is built and timed on the actual batch (graph launches, HIP events);
+the winner is cached per batch signature (system count and log2 bins of
+frame and atom counts) in memory and in a JSON file
+(``$MDTRAJ_AMD_TUNE_CACHE``, default ``~/.cache/mdtraj_amd/autotune.json``).
 """
 from __future__ import annotations
 
@@ -20,8 +22,9 @@ import os
 
 from . import _device as dev
 
-CANDIDATES = ([(fpu, kib, 3) for fpu in (8, 16, 32, 64) for kib in (0, 64, 192)]
-              + [(16, 0, 4), (8, 64, 4)])
+CANDIDATES = ([(64, 0, 3, 3)]
+              + [(fpu, kib, 3, 1) for fpu in (8, 16, 32, 64) for kib in (0, 64, 192)]
+              + [(16, 0, 4, 1), (8, 64, 4, 1)])
 _memory = {}
 
 
@@ -36,7 +39,7 @@ def signature(xyz_list):
     for x in xyz_list:
         k = (int(math.log2(max(1, int(x.shape[0])))), int(math.log2(int(x.shape[1]))))
         bins[k] = bins.get(k, 0) + 1
-    return f"batch-v2:{len(xyz_list)}:" + ",".join(f"{a}.{b}x{c}" for (a, b), c in sorted(bins.items()))
+    return f"batch-v3:{len(xyz_list)}:" + ",".join(f"{a}.{b}x{c}" for (a, b), c in sorted(bins.items()))
 
 
 def _load_disk():
@@ -75,13 +78,15 @@ def time_plan(batch, reps=5):
 
 
 def tune_batch(xyz_list, ref_frames=None, candidates=CANDIDATES, reps=5):
-    """Time every candidate; returns (best (fpu, kib, waves), {candidate: ms})."""
+    """Time every candidate; returns (best (fpu, kib, waves, schedule),
+    {candidate: ms})."""
     from .batch import BatchRMSD
 
     results = {}
-    for fpu, kib, waves in candidates:
-        b = BatchRMSD(xyz_list, ref_frames, frames_per_unit=fpu, unit_kib=kib, waves=waves)
-        results[(fpu, kib, waves)] = time_plan(b, reps)
+    for fpu, kib, waves, schedule in candidates:
+        b = BatchRMSD(xyz_list, ref_frames, frames_per_unit=fpu, unit_kib=kib, waves=waves,
+                      schedule=schedule)
+        results[(fpu, kib, waves, schedule)] = time_plan(b, reps)
         b.close()
     best = min(results, key=results.get)
     return best, results

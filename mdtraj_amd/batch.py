@@ -3,38 +3,45 @@
 The reference scores one trajectory per ``md.rmsd`` call; a batch of uneven
 systems (tens to thousands of atoms and frames each) is a loop of small calls
 there.  Here a plan (``hr_batch_plan_create``) captures a hipGraph whose
-persistent kernel drains a table of (system, frame chunk) work units, largest
-first, so a whole batch is one graph launch with no per-system launch cost.
+scoring kernel covers every system's frames (by default one cost-balanced
+range per workgroup, each system's reference staged in LDS), so a whole
+batch is one graph launch with no per-system launch cost.
 """
 from __future__ import annotations
 
 import ctypes
+import threading
 
 from . import _device as dev
-from ._lib import BatchShape, SystemDesc, call, load
+from ._lib import HR_TUNE_BATCH_SCHEDULE, BatchShape, SystemDesc, call, load
+
+_SCHEDULE_LOCK = threading.Lock()
 
 
 class BatchRMSD:
     """RMSD of every frame of each system to that system's ``ref_frames[s]``.
 
     ``systems``: list of float32 (F_s, N_s, 3) arrays or CUDA tensors (moved
-    to the device once).  Work-unit shape: ``frames_per_unit`` (1..64),
-    ``unit_kib`` (if > 0, fewer frames per unit, a multiple of 4, where
-    frames_per_unit frames would exceed that many KiB) and ``waves`` (3 or 4
-    waves per SIMD of the scoring kernel); None picks the autotuned value for
-    this batch (``mdtraj_amd.autotune``, which times with torch events).
+    to the device once).  ``schedule``: the work split (``HR_TUNE_BATCH_SCHEDULE``:
+    3 one cost-balanced range per workgroup with the reference in LDS, 1 work
+    units from a ticket, 2 one range per wave).  Work-unit shape of schedule
+    1: ``frames_per_unit`` (1..64) and ``unit_kib`` (if > 0, fewer frames per
+    unit, a multiple of 4, where frames_per_unit frames would exceed that
+    many KiB); ``waves``: 3 or 4 waves per SIMD of the scoring kernel.  None
+    picks the autotuned value for this batch (``mdtraj_amd.autotune``, which
+    times with torch events).
     ``run()`` is one graph launch and returns the list of float32 device
     arrays of RMSDs: CUDA tensors on torch's current stream for tensor
     systems, DeviceArrays on the default stream otherwise.
     """
 
     def __init__(self, systems, ref_frames=None, frames_per_unit=None, waves=None,
-                 unit_kib=None):
+                 unit_kib=None, schedule=None):
         load()
         with dev.using(*systems):
-            self._init(systems, ref_frames, frames_per_unit, waves, unit_kib)
+            self._init(systems, ref_frames, frames_per_unit, waves, unit_kib, schedule)
 
-    def _init(self, systems, ref_frames, frames_per_unit, waves, unit_kib):
+    def _init(self, systems, ref_frames, frames_per_unit, waves, unit_kib, schedule):
         self.xyz = [dev.to_device_f32(x) for x in systems]
         for i, x in enumerate(self.xyz):
             if x.dim() != 3 or x.shape[2] != 3:
@@ -49,13 +56,16 @@ class BatchRMSD:
         if frames_per_unit is None or waves is None or unit_kib is None:
             from .autotune import tuned_shape
 
-            t_fpu, t_kib, t_waves = tuned_shape(self.xyz, self.ref_frames)
+            t_fpu, t_kib, t_waves, t_sched = tuned_shape(self.xyz, self.ref_frames)
             frames_per_unit = t_fpu if frames_per_unit is None else frames_per_unit
             unit_kib = t_kib if unit_kib is None else unit_kib
             waves = t_waves if waves is None else waves
+            schedule = t_sched if schedule is None else schedule
         self.frames_per_unit = int(frames_per_unit)
         self.unit_kib = int(unit_kib)
         self.waves = int(waves)
+        # None: the process-wide HR_TUNE_BATCH_SCHEDULE as it stands
+        self.schedule = None if schedule is None else int(schedule)
         self._plan = None
         self._build()
 
@@ -70,8 +80,19 @@ class BatchRMSD:
         plan = ctypes.c_void_p()
         descs = self._descs()
         shape = BatchShape(self.frames_per_unit, self.unit_kib, self.waves)
-        call("hr_batch_plan_create", ctypes.addressof(descs), len(self.xyz),
-             ctypes.addressof(shape), ctypes.byref(plan))
+        args = (ctypes.addressof(descs), len(self.xyz), ctypes.addressof(shape),
+                ctypes.byref(plan))
+        if self.schedule is None:
+            call("hr_batch_plan_create", *args)
+        else:
+            with _SCHEDULE_LOCK:
+                # the work split is a process-wide knob read at plan creation
+                prev = int(load().hr_get_tuning(HR_TUNE_BATCH_SCHEDULE))
+                call("hr_set_tuning", HR_TUNE_BATCH_SCHEDULE, self.schedule)
+                try:
+                    call("hr_batch_plan_create", *args)
+                finally:
+                    call("hr_set_tuning", HR_TUNE_BATCH_SCHEDULE, prev)
         self._plan = plan
 
     @property

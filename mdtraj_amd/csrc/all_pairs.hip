@@ -592,7 +592,7 @@ struct HalfRing {
 };
 constexpr int kHMaxLoads = 10;
 static_assert(HalfRing<25>::kLoads == 38 && HalfRing<26>::kLoads == 39, "load plans");
-constexpr size_t kHalfLds = HalfRing<26>::kLds;  // dynamic LDS of either ring: 79,872 B
+static_assert(HalfRing<26>::kLds == 79872, "dynamic LDS of the 26-slot ring");
 
 template <typename Half, int kS>
 __device__ __forceinline__ void plan_loads_half(const GramArgs<Half>& g, long long i0,

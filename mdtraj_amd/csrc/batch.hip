@@ -39,8 +39,12 @@ struct hr_batch_plan {
     void* d_units = nullptr;
     unsigned* d_ticket = nullptr;
     long long* d_prefix = nullptr;  // static split: cost before system s, s = 0..n
+    std::vector<long long> h_prefix;
+    int* d_first = nullptr;         // block split: first system of each workgroup
     long long total_cost = 0;
-    int schedule = 1;               // 1 ticket, 2 static split
+    int schedule = 3;               // 1 ticket, 2 static split per wave, 3 per workgroup
+    int lds_atoms = 0;              // block split: largest reference staged in LDS
+    int max_atoms = 0;              // largest system with frames
     char* d_ref = nullptr;
     int blocks = 0;
     size_t lds = 0;
@@ -134,14 +138,39 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_kernel(
     }
 }
 
-// Cost of one frame in the static split: its 12 N bytes plus the per-frame
-// work that does not scale with N (row reduction, QCP), which on MI355X costs
-// about as much as streaming 240 more bytes (one-vs-all sweep: ~40 ps per
-// frame + ~2 ps per atom).
-constexpr long long kFrameCost = 240;
+// Cost of one frame in the static splits: its 12 N bytes plus the per-frame
+// work that does not scale with N (row reduction, hand-off of the sums, QCP),
+// which on MI355X costs about as much as streaming 600 more bytes.  Measured
+// with the per-workgroup split on single systems of 10 - 4,400 atoms (the
+// time difference between 0.3 GB and 0.9 GB of frames): ~2.07 ps per atom
+// (5.8 TB/s) plus 45 - 130 ps per frame.
+constexpr long long kFrameCost = 600;
 
 __host__ __device__ __forceinline__ long long frame_cost(int n_atoms) {
     return 12LL * n_atoms + kFrameCost;
+}
+
+// Cost of taking up one more system in a workgroup's or wave's range: its
+// descriptor, reference staging and first loads are latency the frames do
+// not hide (about 4 us of streaming).  Without it a range of many small
+// systems is balanced by bytes alone and finishes last.
+constexpr long long kSystemCost = 24000;
+
+// The frames [*fa, *fb) of a system whose cost starts at `start` (frame f at
+// start + kSystemCost + f * frame_cost) that start in [c0, c1).
+__device__ __forceinline__ void frames_in(long long start, long long n_frames, int n_atoms,
+                                          long long c0, long long c1, long long* fa,
+                                          long long* fb) {
+    const long long pfc = frame_cost(n_atoms);
+    const long long s0 = start + kSystemCost;
+    *fa = c0 > s0 ? min(n_frames, (c0 - s0 + pfc - 1) / pfc) : 0;
+    *fb = c1 > s0 ? min(n_frames, (c1 - s0 + pfc - 1) / pfc) : 0;
+}
+
+// Start of part g of `total` split into n near-equal parts.
+__host__ __device__ __forceinline__ long long split_point(long long total, long long n,
+                                                          long long g) {
+    return total / n * g + total % n * g / n;
 }
 
 // One system's frames fa .. fb through the one-vs-all item loop, the row
@@ -169,7 +198,7 @@ __device__ __forceinline__ void static_segment(const DevSystem& d, long long fa,
 }
 
 // Static split: the frames of all systems, in order, form one sequence whose
-// frame (s, f) starts at cost prefix[s] + f * frame_cost(N_s); wave w of W
+// frame (s, f) starts at cost prefix[s] + kSystemCost + f * frame_cost(N_s); wave w of W
 // owns the frames starting in [C w / W, C (w + 1) / W), C the total cost -
 // one contiguous range per wave, as the one-vs-all kernel splits a single
 // trajectory, balanced by bytes plus per-frame work, with no atomics.
@@ -195,10 +224,89 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_static_kernel(
     }
     for (int si = lo; si < n_systems && prefix[si] < c1; ++si) {
         const DevSystem d = sys[si];
-        const long long pfc = frame_cost(d.n_atoms);
-        const long long fa = min(d.n_frames, c0 > prefix[si] ? (c0 - prefix[si] + pfc - 1) / pfc : 0);
-        const long long fb = min(d.n_frames, (c1 - prefix[si] + pfc - 1) / pfc);
+        long long fa, fb;
+        frames_in(prefix[si], d.n_frames, d.n_atoms, c0, c1, &fa, &fb);
         if (fa < fb) static_segment(d, fa, fb, s_res, lane);
+    }
+}
+
+// Block split (the default): as the static split, but per workgroup, so
+// that the workgroup's four waves share one system at a time and its
+// reference can sit in LDS, as in the one-vs-all kernel.  Workgroup g of G
+// owns the frames starting in [C g / G, C (g + 1) / G); for each system in
+// that range the workgroup copies the system's reference planes to LDS and
+// its waves split the system's frames there into contiguous runs of whole
+// frame sets.  A reference larger than lds_atoms atoms is staged in part
+// (its first lds_atoms atoms) and read from global memory past that.  Frame
+// sums stay in registers (process_chunk_regs), so LDS holds the reference
+// alone.
+template <int kMinWaves>
+__global__ __launch_bounds__(kBlock, kMinWaves) void batch_block_kernel(
+    const DevSystem* __restrict__ sys, const long long* __restrict__ prefix,
+    const int* __restrict__ first, int n_systems, long long total, int lds_atoms) {
+    extern __shared__ float s_ref[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long g = blockIdx.x, n_groups = gridDim.x;
+    const long long c0 = split_point(total, n_groups, g);
+    const long long c1 = split_point(total, n_groups, g + 1);
+    const int lo = first[g];  // last system starting at or before c0 (host table)
+    // every condition below depends only on the workgroup's range, so all
+    // four waves take the same path through the barriers
+    // the next system's descriptor is loaded while the current one runs
+    DevSystem d = sys[lo];
+    long long start = prefix[lo];
+    for (int si = lo; si < n_systems && start < c1; ++si) {
+        const int sn = min(si + 1, n_systems - 1);
+        const DevSystem d_next = sys[sn];
+        const long long start_next = si + 1 < n_systems ? prefix[sn] : c1;
+        long long fa, fb;
+        frames_in(start, d.n_frames, d.n_atoms, c0, c1, &fa, &fb);
+        if (fa >= fb) {
+            d = d_next;
+            start = start_next;
+            continue;
+        }
+        const int n = d.n_atoms;
+        // the reference planes, or their first m atoms, in LDS (stride m),
+        // copied once every wave's first frame loads are out
+        const int m = min(n, lds_atoms);
+        auto stage = [&]() {
+            if (m > 0) {
+                __syncthreads();  // the previous system's reference is no longer read
+                stage_words(s_ref, d.ref_sel, m);
+                stage_words(s_ref + m, d.ref_sel + n, m);
+                stage_words(s_ref + 2 * m, d.ref_sel + 2 * n, m);
+                __syncthreads();
+            }
+        };
+        const GlobalDouble* st = global_ptr(d.ref_stats);
+        const double rs[4] = {st[0], st[1], st[2], st[3]};
+        const GlobalFloat* gx = global_ptr(d.ref_sel);
+#define HR_BLOCK_SEGMENT(RL)                                                                  \
+    do {                                                                                      \
+        constexpr long long kRows = kWave / (RL);                                             \
+        const long long n_sets = (fb - fa + kRows - 1) / kRows;                               \
+        const long long wa = min(fb, fa + kRows * (wv * n_sets / kWavesPerBlock));            \
+        const long long wb = min(fb, fa + kRows * ((wv + 1) * n_sets / kWavesPerBlock));      \
+        if (m == n)                                                                           \
+            process_chunk_regs<(RL), false>(d.xyz, wa, wb, n, s_ref, s_ref + m, s_ref + 2 * m, \
+                                            m, gx, gx + n, gx + 2 * n, rs,                    \
+                                            (long long)d.ref_frame, d.out, lane, stage);      \
+        else                                                                                  \
+            process_chunk_regs<(RL), true>(d.xyz, wa, wb, n, s_ref, s_ref + m, s_ref + 2 * m,  \
+                                           m, gx, gx + n, gx + 2 * n, rs,                     \
+                                           (long long)d.ref_frame, d.out, lane, stage);       \
+    } while (0)
+        switch (auto_row_lanes(n, false)) {
+            case 4: HR_BLOCK_SEGMENT(4); break;
+            case 8: HR_BLOCK_SEGMENT(8); break;
+            case 16: HR_BLOCK_SEGMENT(16); break;
+            default: HR_BLOCK_SEGMENT(32); break;
+        }
+#undef HR_BLOCK_SEGMENT
+        d = d_next;
+        start = start_next;
     }
 }
 
@@ -213,35 +321,73 @@ void release(hr_batch_plan* p) {
     if (p->d_units) (void)hipFree(p->d_units);
     if (p->d_ticket) (void)hipFree(p->d_ticket);
     if (p->d_prefix) (void)hipFree(p->d_prefix);
+    if (p->d_first) (void)hipFree(p->d_first);
     if (p->d_ref) (void)hipFree(p->d_ref);
     delete p;
 }
 
 template <int kMinWaves>
 int capture_plan(hr_batch_plan* p) {
-    const bool fixed = p->schedule == 2;
-    const void* k = fixed ? reinterpret_cast<const void*>(batch_static_kernel<kMinWaves>)
-                          : reinterpret_cast<const void*>(batch_kernel<kMinWaves>);
-    const size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
-                       sizeof(double);
-    int per_cu = 0;
-    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
+    const int sched = p->schedule;
+    const void* k = sched == 3   ? reinterpret_cast<const void*>(batch_block_kernel<kMinWaves>)
+                    : sched == 2 ? reinterpret_cast<const void*>(batch_static_kernel<kMinWaves>)
+                                 : reinterpret_cast<const void*>(batch_kernel<kMinWaves>);
+    size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
+                 sizeof(double);
+    if (sched == 3) {
+        // the largest reference, up to what leaves room for kMinWaves
+        // workgroups per CU.  A workgroup's LDS is allocated in granules:
+        // 54,576 B passes the occupancy query for three per CU on MI355X,
+        // yet a third of such a grid waits for a second round, so the share
+        // is rounded down to whole KiB first.
+        const size_t per_group =
+            std::min(d->lds_per_cu / kMinWaves, d->lds_per_block) / 1024 * 1024;
+        const long long fit = (long long)per_group / 12 / 4 * 4;
+        p->lds_atoms = (int)std::min<long long>(p->max_atoms, fit);
+        for (;;) {
+            int per_cu = 0;
+            lds = (size_t)12 * p->lds_atoms;
+            HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
+            if (per_cu >= kMinWaves || p->lds_atoms == 0) break;
+            p->lds_atoms = std::max(0, p->lds_atoms - 64);
+        }
+    }
+    int per_cu = 0;
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
     long long blocks = (long long)(per_cu < 1 ? 1 : per_cu) * d->cu_count;
-    // ticket: no more waves than units; static split: no more than frames
-    const long long work = fixed ? p->total_frames : p->n_units;
+    // ticket: no more waves than units; static splits: no more than frames
+    const long long work = sched == 1 ? p->n_units : p->total_frames;
     const long long by_work = (work + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks > by_work) blocks = by_work;
     if (blocks < 1) blocks = 1;
     p->blocks = (int)blocks;
     p->lds = lds;
+    if (sched == 3) {
+        // each workgroup's first system: the last one starting at or before
+        // the workgroup's first cost
+        std::vector<int> first(p->blocks);
+        for (int g = 0; g < p->blocks; ++g) {
+            const long long c0 = split_point(p->total_cost, p->blocks, g);
+            const auto it = std::upper_bound(p->h_prefix.begin(),
+                                             p->h_prefix.begin() + p->n_systems, c0);
+            first[g] = std::max(0, (int)(it - p->h_prefix.begin()) - 1);
+        }
+        HR_CHECK_HIP(hipMalloc(&p->d_first, sizeof(int) * first.size()));
+        HR_CHECK_HIP(hipMemcpy(p->d_first, first.data(), sizeof(int) * first.size(),
+                               hipMemcpyHostToDevice));
+    }
     hipStream_t s = p->capture;
     HR_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    if (!fixed) hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
+    if (sched == 1) hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
     hipLaunchKernelGGL(batch_ref_prep_kernel, dim3(p->n_systems), dim3(kBlock), 0, s,
                        static_cast<const DevSystem*>(p->d_systems));
-    if (fixed)
+    if (sched == 3)
+        hipLaunchKernelGGL(batch_block_kernel<kMinWaves>, dim3(p->blocks), dim3(kBlock), lds, s,
+                           static_cast<const DevSystem*>(p->d_systems), p->d_prefix, p->d_first,
+                           p->n_systems, p->total_cost, p->lds_atoms);
+    else if (sched == 2)
         hipLaunchKernelGGL(batch_static_kernel<kMinWaves>, dim3(p->blocks), dim3(kBlock), lds, s,
                            static_cast<const DevSystem*>(p->d_systems), p->d_prefix,
                            p->n_systems, p->total_cost);
@@ -296,12 +442,15 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
     p->device = d->device;
     p->n_systems = n_systems;
     p->frames_per_unit = frames_per_unit;
-    p->schedule = tuning(HR_TUNE_BATCH_SCHEDULE) == 2 ? 2 : 1;
+    const int sched = tuning(HR_TUNE_BATCH_SCHEDULE);
+    p->schedule = sched == 0 ? 3 : sched;
     // static split: cost of the frames before each system
     std::vector<long long> prefix(n_systems + 1, 0);
     for (int i = 0; i < n_systems; ++i)
-        prefix[i + 1] = prefix[i] + systems[i].n_frames * frame_cost(systems[i].n_atoms);
+        prefix[i + 1] = prefix[i] + (systems[i].n_frames > 0 ? kSystemCost : 0) +
+                        systems[i].n_frames * frame_cost(systems[i].n_atoms);
     p->total_cost = prefix[n_systems];
+    p->h_prefix = prefix;
 
     // reference workspace: per system planes (3 N floats) then 4 doubles
     std::vector<size_t> ref_off(n_systems);
@@ -325,6 +474,7 @@ int hr_batch_plan_create(const hr_system_desc* systems, int32_t n_systems,
             units.push_back(DevUnit{i, (int)c, b});
         }
         p->total_frames += systems[i].n_frames;
+        if (systems[i].n_frames > 0) p->max_atoms = std::max(p->max_atoms, systems[i].n_atoms);
         p->total_bytes += 12.0 * (double)systems[i].n_frames * systems[i].n_atoms;
     }
     std::stable_sort(units.begin(), units.end(), [&](const DevUnit& a, const DevUnit& b) {

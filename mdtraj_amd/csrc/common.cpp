@@ -39,14 +39,18 @@ const DeviceInfo* current_device() {
     if (d.device < 0) {
         int cu = 0;
         int lds = 0;
+        int lds_cu = 0;
         if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
-                hipSuccess) {
+                hipSuccess ||
+            hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  dev) != hipSuccess) {
             set_error("hipDeviceGetAttribute failed on device %d", dev);
             return nullptr;
         }
         d.cu_count = cu > 0 ? cu : 1;
         d.lds_per_block = lds > 0 ? (size_t)lds : 65536;
+        d.lds_per_cu = lds_cu > 0 ? (size_t)lds_cu : d.lds_per_block;
         d.device = dev;
     }
     return &d;
@@ -85,8 +89,8 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_FRAMES_ROW_LANES must be 0, 4, 8, 16, 32 or 64 (got %d)", value);
         return HR_ERR_INVALID;
     }
-    if (knob == HR_TUNE_BATCH_SCHEDULE && value != 0 && value != 1 && value != 2) {
-        hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0, 1 or 2 (got %d)", value);
+    if (knob == HR_TUNE_BATCH_SCHEDULE && (value < 0 || value > 3)) {
+        hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0 .. 3 (got %d)", value);
         return HR_ERR_INVALID;
     }
     if (knob == HR_TUNE_PAIRS_TILE && (value < 0 || value > 5)) {

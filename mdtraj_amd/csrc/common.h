@@ -18,6 +18,7 @@ struct DeviceInfo {
     int device = -1;
     int cu_count = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
 };
 // Properties of the current device (initialised on first use per device).
 const DeviceInfo* current_device();

@@ -201,6 +201,47 @@ constexpr size_t result_lds_bytes() {
     return (size_t)kWavesPerBlock * kWave * SumsPerFrame<kKind>::value * sizeof(double);
 }
 
+// Copies n 32-bit words from global memory to LDS, the workgroup's lanes
+// each issuing up to kStageBatch loads before the first LDS store, so that a
+// reference of thousands of atoms costs a couple of load latencies rather
+// than one per 256 words (a plain copy loop waits for each load before its
+// store).  16-byte loads when both pointers allow them.  No barrier.
+constexpr int kStageBatch = 8;
+
+template <typename T>
+__device__ __forceinline__ void stage_words(T* __restrict__ dst, const T* __restrict__ src,
+                                            int n) {
+    static_assert(sizeof(T) == 4, "32-bit words");
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const float* s = reinterpret_cast<const float*>(src);
+    float* d = reinterpret_cast<float*>(dst);
+    int done = 0;
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0) {
+        const int n4 = n / 4;
+        const __attribute__((address_space(1))) v4f* s4 =
+            reinterpret_cast<const __attribute__((address_space(1))) v4f*>(global_ptr(s));
+        v4f* d4 = reinterpret_cast<v4f*>(d);
+        for (int j0 = threadIdx.x; j0 < n4; j0 += kBlock * kStageBatch) {
+            v4f v[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) v[u] = s4[min(j0 + u * kBlock, n4 - 1)];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+                if (j0 + u * kBlock < n4) d4[j0 + u * kBlock] = v[u];
+        }
+        done = 4 * n4;
+    }
+    const GlobalFloat* g = global_ptr(s);
+    for (int j0 = done + (int)threadIdx.x; j0 < n; j0 += kBlock * kStageBatch) {
+        float v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) v[u] = g[min(j0 + u * kBlock, n - 1)];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u)
+            if (j0 + u * kBlock < n) d[j0 + u * kBlock] = v[u];
+    }
+}
+
 // Row layout: a row of kRL lanes (4, 8, 16, 32 or 64) owns one frame, so a
 // wave works on kWave / kRL frames at once (row r = lane / kRL holds frame
 // base + r).  An item is (frame set q, atom block b): lane t of a row takes
@@ -473,6 +514,144 @@ __device__ __forceinline__ void process_chunk(
             prime_ring<kAxis, kGather, kRL, kRefRing>(
                 xyz, next, (int)min((long long)kWave, f_end - next), frame_stride, rowstride, ix,
                 n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
+    }
+}
+
+// process_chunk's item ring for RMSDs of whole frames (atom-major, no
+// gather), with each group's frame sums kept in registers instead of parked
+// in LDS: after frame set q's row reduction, lane k of the wave takes frame
+// base + k's sums from the first lane of row k mod kRows (13 doubles by
+// permutes), so lane k finishes frame base + k from its own registers.  The
+// workgroup's LDS then holds only the reference, which lets the batched
+// kernel stage references of up to ~4,500 atoms at 3 waves/SIMD.  The
+// reference is read at accumulation time from LDS (planes lx, ly, lz).
+// kPartial: only atoms below n_lds are in LDS (a reference larger than the
+// workgroup's share), and items past them read gx, gy, gz in global memory,
+// where each read waits for the frame loads issued before it (a reference
+// riding the load ring as in process_chunk would spill registers beside the
+// sums kept here).
+template <int kRL>
+__device__ __forceinline__ void prime_ring_regs(const float* __restrict__ xyz, long long base,
+                                                int cnt, int n_sel, int n_blk, int row, int t,
+                                                float (&bx)[kRing][kItemAtoms],
+                                                float (&by)[kRing][kItemAtoms],
+                                                float (&bz)[kRing][kItemAtoms]) {
+    constexpr int kRows = kWave / kRL;
+    const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+#pragma unroll
+    for (int r = 0; r < kRing - 1; ++r) {
+        const int rc = min(r, n_items - 1);
+        const int q = rc / n_blk, b = rc - q * n_blk;
+        const int k = kRows * q + row;
+        load_item<false, false, kRL, true>(xyz + (base + (k < cnt ? k : 0)) * 3LL * n_sel, 0,
+                                           nullptr, n_sel, t, b, bx[r], by[r], bz[r]);
+    }
+}
+
+// The item loop runs a multiple of kRing steps with no branch around a load
+// (as run_group_batch): every step refills the slot freed one step ago, an
+// item past the group's end reloading the last one, and steps past the last
+// item accumulate zeros.  Loads under a branch are merged at its join by
+// register copies that wait for them at once, which leaves one item in
+// flight instead of kRing - 1.
+template <int kRL, bool kPartial, typename Stage>
+__device__ __forceinline__ void process_chunk_regs(
+    const float* __restrict__ xyz, long long f_begin, long long f_end, int n_sel,
+    const float* __restrict__ lx, const float* __restrict__ ly, const float* __restrict__ lz,
+    int n_lds, const GlobalFloat* __restrict__ gx, const GlobalFloat* __restrict__ gy,
+    const GlobalFloat* __restrict__ gz, const double (&rs)[4], long long self_frame,
+    float* __restrict__ out, int lane, Stage&& stage) {
+    constexpr int kRows = kWave / kRL;
+    const int row = lane / kRL, t = lane % kRL;
+    const long long frame_stride = 3LL * n_sel;
+    const double n = (double)n_sel;
+    const int n_blk = (n_sel + kRL * kItemAtoms - 1) / (kRL * kItemAtoms);
+    float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
+    if (f_begin < f_end)
+        prime_ring_regs<kRL>(xyz, f_begin, (int)min((long long)kWave, f_end - f_begin), n_sel,
+                             n_blk, row, t, bx, by, bz);
+    // the caller's staging of the reference (every wave, barriers included),
+    // while the first frame loads are in flight
+    stage();
+    for (long long base = f_begin; base < f_end; base += kWave) {
+        const int cnt = (int)min((long long)kWave, f_end - base);
+        const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+        const int n_steps = (n_items + kRing - 1) / kRing * kRing;
+        Sums s = {}, mine = {};
+        for (int it0 = 0; it0 < n_steps; it0 += kRing) {
+#pragma unroll
+            for (int r = 0; r < kRing; ++r) {
+                const int it = it0 + r;
+                {
+                    const int nxt = min(it + kRing - 1, n_items - 1);
+                    const int q = nxt / n_blk, b = nxt - q * n_blk;
+                    const int k = kRows * q + row;
+                    const int slot = (r + kRing - 1) % kRing;
+                    load_item<false, false, kRL, true>(
+                        xyz + (base + (k < cnt ? k : 0)) * frame_stride, 0, nullptr, n_sel, t, b,
+                        bx[slot], by[slot], bz[slot]);
+                }
+                const bool live = it < n_items;  // wave-uniform
+                const int itc = live ? it : n_items - 1;
+                const int q = itc / n_blk, b = itc - q * n_blk;
+                const int j0 = b * kRL * kItemAtoms + t;
+                const bool row_ok = live && kRows * q + row < cnt;
+                // reference atoms of the item: from LDS, or past the staged
+                // part (kPartial: atoms n_lds and up) from global memory
+                float ux[kItemAtoms], uy[kItemAtoms], uz[kItemAtoms];
+                const bool from_lds = !kPartial || (b + 1) * kRL * kItemAtoms <= n_lds;
+                if (from_lds) {
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int jc = min(j0 + u * kRL, n_sel - 1);
+                        ux[u] = lx[jc];
+                        uy[u] = ly[jc];
+                        uz[u] = lz[jc];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int jc = min(j0 + u * kRL, n_sel - 1);
+                        ux[u] = gx[jc];
+                        uy[u] = gy[jc];
+                        uz[u] = gz[jc];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kItemAtoms; ++u) {
+                    const int j = j0 + u * kRL;
+                    const bool ok = row_ok && j < n_sel;
+                    accumulate<Kind::Rmsd>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                           valid_atom(ok, bz[r][u]), ux[u], uy[u], uz[u]);
+                }
+                if (live && b == n_blk - 1) {  // wave-uniform: the set's last block
+                    row_reduce<Kind::Rmsd, kRL>(s);
+                    const int src = (lane % kRows) * kRL;
+                    Sums got;
+                    got.q = __shfl(s.q, src, kWave);
+                    got.sx = __shfl(s.sx, src, kWave);
+                    got.sy = __shfl(s.sy, src, kWave);
+                    got.sz = __shfl(s.sz, src, kWave);
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) got.p[e] = __shfl(s.p[e], src, kWave);
+                    take<Kind::Rmsd>(mine, got, lane / kRows == q);
+                    s = Sums{};
+                }
+            }
+        }
+        // the next group's first loads go out before this group's QCP solves
+        // (unconditional: past the range the last group is loaded again)
+        const long long next = base + kWave;
+        const long long nb = next < f_end ? next : base;
+        prime_ring_regs<kRL>(xyz, nb, (int)min((long long)kWave, f_end - nb), n_sel, n_blk, row,
+                             t, bx, by, bz);
+        if (lane < cnt) {
+            const long long f = base + lane;
+            double M[9], ga, gb;
+            centred(mine, rs, n, M, ga, gb);
+            const double msd = qcp_msd(M, ga, gb, n);
+            global_out(out)[f] = (f == self_frame) ? 0.0f : (float)sqrt(msd);
+        }
     }
 }
 

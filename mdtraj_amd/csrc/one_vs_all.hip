@@ -48,14 +48,8 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
         float* s_ry = smem + n_sel;
         float* s_rz = smem + 2 * n_sel;
         int* s_idx = reinterpret_cast<int*>(kStage == kStageAll ? smem + 3 * n_sel : smem);
-        for (int j = threadIdx.x; j < n_sel; j += kBlock) {
-            if constexpr (kStage == kStageAll) {
-                s_rx[j] = ref_sel[j];
-                s_ry[j] = ref_sel[n_sel + j];
-                s_rz[j] = ref_sel[2 * n_sel + j];
-            }
-            if constexpr (kGather) s_idx[j] = idx[j];
-        }
+        if constexpr (kStage == kStageAll) stage_words(s_rx, ref_sel, 3 * n_sel);
+        if constexpr (kGather) stage_words(s_idx, idx, n_sel);
         __syncthreads();
         if constexpr (kStage == kStageAll) {
             rx = s_rx;

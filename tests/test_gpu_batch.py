@@ -23,12 +23,13 @@ def systems(seed=0, n=24):
     return out
 
 
-@pytest.mark.parametrize("schedule", [1, 2])
+@pytest.mark.parametrize("schedule", [1, 2, 3])
 @pytest.mark.parametrize("fpu,kib,waves", [(64, 0, 3), (16, 0, 3), (7, 0, 4), (64, 0, 4),
                                            (64, 8, 3), (8, 64, 3)])
 def test_batch_vs_fp64(gpu, fpu, kib, waves, schedule):
-    """Both work splits (ticket units, static cost split) over systems of
-    1 - 1,300 atoms and 1 - 139 frames."""
+    """Every work split (ticket units, static cost split per wave, per
+    workgroup with the reference in LDS) over systems of 1 - 1,300 atoms and
+    1 - 139 frames."""
     from mdtraj_amd import _lib
 
     xs = systems()
@@ -42,6 +43,25 @@ def test_batch_vs_fp64(gpu, fpu, kib, waves, schedule):
     for x, r, got in zip(xs, refs, res):
         want = fp64.rmsd(x, x, r)
         assert got.shape == want.shape
+        assert np.abs(got - want).max() < 1e-5
+        assert got[r] == 0.0
+    b.close()
+
+
+@pytest.mark.parametrize("waves", [3, 4])
+def test_block_split_reference_beyond_lds(gpu, waves):
+    """The per-workgroup split stages references of up to ~4,500 atoms (3
+    waves/SIMD) or ~3,400 (4) in LDS; larger systems read theirs from global
+    memory.  Mixed in one plan, both paths match the FP64 oracle."""
+    xs = []
+    for s, (F, N) in enumerate([(9, 3500), (70, 40), (5, 5000), (33, 4400), (3, 9000), (65, 1)]):
+        x, _ = gen.small_cube(seed=2000 + s, n_frames=F, n_atoms=N, sigma=0.03)
+        xs.append(x)
+    refs = [x.shape[0] // 2 for x in xs]
+    b = BatchRMSD(xs, refs, waves=waves)
+    res = [dev.to_host(r) for r in b.run()]
+    for x, r, got in zip(xs, refs, res):
+        want = fp64.rmsd(x, x, r)
         assert np.abs(got - want).max() < 1e-5
         assert got[r] == 0.0
     b.close()
@@ -70,9 +90,10 @@ def test_autotuner_candidates_agree(gpu):
     from mdtraj_amd import autotune
 
     xs = systems(seed=5, n=12)
-    best, results = autotune.tune_batch(xs, candidates=[(16, 0, 3), (64, 32, 4)], reps=2)
+    best, results = autotune.tune_batch(xs, candidates=[(16, 0, 3, 1), (64, 32, 4, 1),
+                                                        (64, 0, 3, 3)], reps=2)
     assert best in results and all(v > 0 for v in results.values())
-    a = BatchRMSD(xs, frames_per_unit=16, unit_kib=0, waves=3).run()
-    b = BatchRMSD(xs, frames_per_unit=64, unit_kib=32, waves=4).run()
+    a = BatchRMSD(xs, frames_per_unit=16, unit_kib=0, waves=3, schedule=1).run()
+    b = BatchRMSD(xs, frames_per_unit=64, unit_kib=32, waves=4, schedule=1).run()
     for u, v in zip(a, b):
         assert np.array_equal(dev.to_host(u), dev.to_host(v))
