@@ -52,11 +52,9 @@ int hr_device_cu_count(void);
  * mdtraj_amd/autotune.py).  Values are process-wide; 0 restores the default.
  *   HR_TUNE_FRAMES_WAVES_PER_SIMD  register budget of the per-frame kernel:
  *                                  3 (default, no spills) or 4 waves/SIMD
- *   HR_TUNE_FRAMES_GRID            grid of the per-frame kernel (whole
- *                                  frame sets per wave either way):
- *                                  2 (default) one wave per resident slot;
- *                                  1 only as many waves as give every wave
- *                                  the same set count
+ *   HR_TUNE_FRAMES_GRID            accepted for compatibility (0, 1, 2);
+ *                                  the per-frame kernel's grid is now one
+ *                                  workgroup per CU
  *   HR_TUNE_FRAMES_ROW_LANES       lanes per frame of the per-frame kernel:
  *                                  0 (default) chosen by selection size,
  *                                  or 4, 8, 16, 32, 64 (16 down to 1 frames
@@ -96,7 +94,19 @@ int hr_device_cu_count(void);
  *                                  an epilogue through LDS (2), 26-slot rows
  *                                  (3), the QCP solved from registers (4),
  *                                  both (5, the default).  Results do not
- *                                  depend on it. */
+ *                                  depend on it.
+ *   HR_TUNE_FRAMES_STATIC_PCT      per-frame kernel (one workgroup per CU):
+ *                                  percent of each workgroup's frame sets
+ *                                  split evenly over its waves up front (0:
+ *                                  the default, 45); the rest goes out in
+ *                                  chunks by a counter in LDS, so that the
+ *                                  waves a SIMD issues less often take fewer
+ *                                  frames; 100: all of them up front.
+ *                                  Results do not depend on it.
+ *   HR_TUNE_FRAMES_CHUNK_KIB       per-frame kernel: about this many KiB of
+ *                                  frames per chunk (0: the default, 32), in
+ *                                  whole frame sets of at most 64 frames.
+ *                                  Results do not depend on it. */
 #define HR_TUNE_FRAMES_WAVES_PER_SIMD 0
 #define HR_TUNE_FRAMES_GRID 1
 #define HR_TUNE_FRAMES_ROW_LANES 2
@@ -104,7 +114,9 @@ int hr_device_cu_count(void);
 #define HR_TUNE_SUPERPOSE_CHUNK_MIB 4
 #define HR_TUNE_HOST_DIRECT_KIB 5
 #define HR_TUNE_PAIRS_TILE 6
-#define HR_TUNE_COUNT 7
+#define HR_TUNE_FRAMES_STATIC_PCT 7
+#define HR_TUNE_FRAMES_CHUNK_KIB 8
+#define HR_TUNE_COUNT 9
 int hr_set_tuning(int knob, int value);
 int hr_get_tuning(int knob);
 

@@ -24,6 +24,8 @@ HR_TUNE_BATCH_SCHEDULE = 3
 HR_TUNE_SUPERPOSE_CHUNK_MIB = 4
 HR_TUNE_HOST_DIRECT_KIB = 5
 HR_TUNE_PAIRS_TILE = 6
+HR_TUNE_FRAMES_STATIC_PCT = 7
+HR_TUNE_FRAMES_CHUNK_KIB = 8
 
 HR_PAIRS_FP32 = 0
 HR_PAIRS_F16X3 = 1

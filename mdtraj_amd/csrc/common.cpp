@@ -97,6 +97,14 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_PAIRS_TILE must be 0 .. 5 (got %d)", value);
         return HR_ERR_INVALID;
     }
+    if (knob == HR_TUNE_FRAMES_STATIC_PCT && (value < 0 || value > 100)) {
+        hr::set_error("HR_TUNE_FRAMES_STATIC_PCT must be 0 .. 100 (got %d)", value);
+        return HR_ERR_INVALID;
+    }
+    if (knob == HR_TUNE_FRAMES_CHUNK_KIB && value < 0) {
+        hr::set_error("HR_TUNE_FRAMES_CHUNK_KIB must be 0 or positive (got %d)", value);
+        return HR_ERR_INVALID;
+    }
     if (knob == HR_TUNE_HOST_DIRECT_KIB && value < -1) {
         hr::set_error("HR_TUNE_HOST_DIRECT_KIB must be -1, 0 or positive (got %d)", value);
         return HR_ERR_INVALID;

@@ -13,6 +13,20 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kDefaultWavesPerCu = 16;
 
+// The per-frame kernels run one workgroup per CU holding all of its waves
+// (kMinWaves per SIMD), so that those waves can share work through LDS.
+template <int kMinWaves>
+struct CuGroup {
+    static constexpr int kWaves = 4 * kMinWaves;
+    static constexpr int kThreads = kWave * kWaves;
+};
+
+// Share of each workgroup's frame sets split statically over its waves (the
+// rest goes out in chunks by an LDS ticket), and the bytes of frames per
+// chunk; HR_TUNE_FRAMES_STATIC_PCT and HR_TUNE_FRAMES_CHUNK_KIB override.
+constexpr int kDefaultStaticPct = 45;
+constexpr int kDefaultChunkKiB = 32;
+
 // One-vs-all workspace: [0, 64) reference sums (double sx, sy, sz, q);
 // [256, 256 + 12 n_sel) gathered reference, axis-major (x[], y[], z[]).
 constexpr size_t kRefOffset = 256;
@@ -195,20 +209,21 @@ __device__ __forceinline__ void load_sums(Sums& s, const double* src) {
     }
 }
 
-// LDS bytes of the per-wave result area (64 frames of sums per wave).
+// LDS bytes of the per-wave result area (64 frames of sums per wave) of a
+// workgroup of `waves` waves.
 template <Kind kKind>
-constexpr size_t result_lds_bytes() {
-    return (size_t)kWavesPerBlock * kWave * SumsPerFrame<kKind>::value * sizeof(double);
+constexpr size_t result_lds_bytes(int waves = kWavesPerBlock) {
+    return (size_t)waves * kWave * SumsPerFrame<kKind>::value * sizeof(double);
 }
 
-// Copies n 32-bit words from global memory to LDS, the workgroup's lanes
-// each issuing up to kStageBatch loads before the first LDS store, so that a
+// Copies n 32-bit words from global memory to LDS, the workgroup's kThreads
+// lanes each issuing up to kStageBatch loads before the first LDS store, so that a
 // reference of thousands of atoms costs a couple of load latencies rather
 // than one per 256 words (a plain copy loop waits for each load before its
 // store).  16-byte loads when both pointers allow them.  No barrier.
 constexpr int kStageBatch = 8;
 
-template <typename T>
+template <int kThreads = kBlock, typename T>
 __device__ __forceinline__ void stage_words(T* __restrict__ dst, const T* __restrict__ src,
                                             int n) {
     static_assert(sizeof(T) == 4, "32-bit words");
@@ -221,24 +236,24 @@ __device__ __forceinline__ void stage_words(T* __restrict__ dst, const T* __rest
         const __attribute__((address_space(1))) v4f* s4 =
             reinterpret_cast<const __attribute__((address_space(1))) v4f*>(global_ptr(s));
         v4f* d4 = reinterpret_cast<v4f*>(d);
-        for (int j0 = threadIdx.x; j0 < n4; j0 += kBlock * kStageBatch) {
+        for (int j0 = threadIdx.x; j0 < n4; j0 += kThreads * kStageBatch) {
             v4f v[kStageBatch];
 #pragma unroll
-            for (int u = 0; u < kStageBatch; ++u) v[u] = s4[min(j0 + u * kBlock, n4 - 1)];
+            for (int u = 0; u < kStageBatch; ++u) v[u] = s4[min(j0 + u * kThreads, n4 - 1)];
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u)
-                if (j0 + u * kBlock < n4) d4[j0 + u * kBlock] = v[u];
+                if (j0 + u * kThreads < n4) d4[j0 + u * kThreads] = v[u];
         }
         done = 4 * n4;
     }
     const GlobalFloat* g = global_ptr(s);
-    for (int j0 = done + (int)threadIdx.x; j0 < n; j0 += kBlock * kStageBatch) {
+    for (int j0 = done + (int)threadIdx.x; j0 < n; j0 += kThreads * kStageBatch) {
         float v[kStageBatch];
 #pragma unroll
-        for (int u = 0; u < kStageBatch; ++u) v[u] = g[min(j0 + u * kBlock, n - 1)];
+        for (int u = 0; u < kStageBatch; ++u) v[u] = g[min(j0 + u * kThreads, n - 1)];
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u)
-            if (j0 + u * kBlock < n) d[j0 + u * kBlock] = v[u];
+            if (j0 + u * kThreads < n) d[j0 + u * kThreads] = v[u];
     }
 }
 
@@ -356,18 +371,38 @@ __device__ __forceinline__ void row_reduce(Sums& s) {
 // Finish a group of cnt frames whose row sums are parked in s_res: one lane
 // per frame computes the QCP MSD (or the fit, or the no-superpose MSD).
 template <Kind kKind>
+__device__ __forceinline__ void finish_group_at(long long f, int cnt, double n,
+                                                const double (&rs)[4], long long self_frame,
+                                                float* __restrict__ out,
+                                                float* __restrict__ rot_out,
+                                                double* __restrict__ mu_out,
+                                                double* __restrict__ s_res, int lane);
+
+template <Kind kKind>
 __device__ __forceinline__ void finish_group(long long base, int cnt, double n,
                                              const double (&rs)[4], long long self_frame,
                                              float* __restrict__ out, float* __restrict__ rot_out,
                                              double* __restrict__ mu_out,
                                              double* __restrict__ s_res, int lane) {
+    finish_group_at<kKind>(base + lane, cnt, n, rs, self_frame, out, rot_out, mu_out, s_res,
+                           lane);
+}
+
+// As finish_group, lane k < cnt finishing frame f (its own frame index: the
+// frames of a group need not be consecutive).
+template <Kind kKind>
+__device__ __forceinline__ void finish_group_at(long long f, int cnt, double n,
+                                                const double (&rs)[4], long long self_frame,
+                                                float* __restrict__ out,
+                                                float* __restrict__ rot_out,
+                                                double* __restrict__ mu_out,
+                                                double* __restrict__ s_res, int lane) {
     constexpr int kNS = SumsPerFrame<kKind>::value;
     // the row leaders' LDS stores must be visible to every lane of the wave
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane < cnt) {
-        const long long f = base + lane;
         Sums mine;
         load_sums<kKind>(mine, s_res + lane * kNS);
         if constexpr (kKind == Kind::NoSuperpose) {
@@ -514,6 +549,127 @@ __device__ __forceinline__ void process_chunk(
             prime_ring<kAxis, kGather, kRL, kRefRing>(
                 xyz, next, (int)min((long long)kWave, f_end - next), frame_stride, rowstride, ix,
                 n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
+    }
+}
+
+// The per-frame item loop over a workgroup that fills a CU.  The waves
+// sharing a SIMD do not progress alike (the oldest is issued first): with a
+// static split of the frames over 3 waves/SIMD (MI355X, 100,000 x 2,000
+// frames) they finish in three bands, at about 270, 315 and 375 us, and the
+// last band sets the kernel's time.  One workgroup per CU holds all of a
+// CU's waves, so they can share its work through an LDS counter: each wave
+// first takes its static range [sa, sb), then chunks of `chunk` frames from
+// [da, db) by the ticket *s_ticket (chunk c = frames da + c chunk ...), until
+// the workgroup's frames run out.  Chunks fill the wave's group of 64 frame
+// slots in turn, lane k keeping the index of the frame in slot k, so the QCP
+// finish still runs once per 64 frames.  The item loop is process_chunk's.
+template <Kind kKind, bool kAxis, bool kGather, int kRL, bool kRefRing>
+__device__ __forceinline__ void process_cu(
+    const float* __restrict__ xyz, long long sa, long long sb, long long da, long long db,
+    long long chunk, unsigned* s_ticket, long long frame_stride, int rowstride,
+    const int* __restrict__ ix, int n_sel, const float* __restrict__ rx,
+    const float* __restrict__ ry, const float* __restrict__ rz, const double (&rs)[4],
+    long long self_frame, float* __restrict__ out, float* __restrict__ rot_out,
+    double* __restrict__ mu_out, double* __restrict__ s_res, int lane) {
+    constexpr int kNS = SumsPerFrame<kKind>::value;
+    constexpr int kRows = kWave / kRL;  // frames side by side in the wave
+    constexpr bool kEarly = kKind != Kind::Fit && !kRefRing;
+    const int row = lane / kRL, t = lane % kRL;
+    const double n = (double)n_sel;
+    const int n_blk = (n_sel + kRL * kItemAtoms - 1) / (kRL * kItemAtoms);
+    float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
+    float qx[kRing][kItemAtoms], qy[kRing][kItemAtoms], qz[kRing][kItemAtoms];  // kRefRing
+    long long ra = sa, rb = sb;
+    // the next chunk of the workgroup's shared frames (empty when none is left)
+    auto take_chunk = [&]() {
+        unsigned c = 0;
+        if (lane == 0) c = atomicAdd(s_ticket, 1u);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
+        ra = da + (long long)c * chunk;
+        rb = ra < db ? min(db, ra + chunk) : ra;
+    };
+    if (ra >= rb) take_chunk();
+    if (ra < rb)
+        prime_ring<kAxis, kGather, kRL, kRefRing>(
+            xyz, ra, (int)min((long long)kWave, rb - ra), frame_stride, rowstride, ix, n_sel, rx,
+            ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
+    int k0 = 0;           // slots of the group filled so far
+    long long my_f = 0;   // frame of this lane's slot
+    while (ra < rb) {
+        const long long base = ra;
+        const int cnt = (int)min((long long)(kWave - k0), rb - ra);
+        double* s_dst = s_res + k0 * kNS;
+        const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+        Sums s = {};
+        for (int it0 = 0; it0 < n_items; it0 += kRing) {
+#pragma unroll
+            for (int r = 0; r < kRing; ++r) {
+                const int it = it0 + r;
+                if (it < n_items) {
+                    // refill the slot freed one step ago with item it + kRing - 1
+                    const int nxt = it + kRing - 1;
+                    if (nxt < n_items) {
+                        const int q = nxt / n_blk, b = nxt - q * n_blk;
+                        const int k = kRows * q + row;
+                        const int slot = (r + kRing - 1) % kRing;  // static once unrolled
+                        load_item<kAxis, kGather, kRL>(
+                            xyz + (base + (k < cnt ? k : 0)) * frame_stride, rowstride, ix,
+                            n_sel, t, b, bx[slot], by[slot], bz[slot]);
+                        if constexpr (kRefRing)
+                            load_ref_item<kRL>(rx, ry, rz, n_sel, t, b, qx[slot], qy[slot],
+                                               qz[slot]);
+                    }
+                    const int q = it / n_blk, b = it - q * n_blk;
+                    const int j0 = b * kRL * kItemAtoms + t;
+                    const bool row_ok = kRows * q + row < cnt;
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int j = j0 + u * kRL;
+                        const bool ok = row_ok && j < n_sel;
+                        const int jc = min(j, n_sel - 1);
+                        constexpr bool kRef = kKind == Kind::NoSuperpose;
+                        const float ux = kRefRing ? qx[r][u] : rx[jc];
+                        const float uy = kRefRing ? qy[r][u] : ry[jc];
+                        const float uz = kRefRing ? qz[r][u] : rz[jc];
+                        accumulate<kKind>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                          valid_atom(ok, bz[r][u]),
+                                          kRef ? valid_atom(ok, ux) : ux,
+                                          kRef ? valid_atom(ok, uy) : uy,
+                                          kRef ? valid_atom(ok, uz) : uz);
+                    }
+                    if (b == n_blk - 1) {
+                        row_reduce<kKind, kRL>(s);
+                        const int k = kRows * q + row;
+                        if (t == 0 && k < cnt) store_sums<kKind>(s_dst + k * kNS, s);
+                        s = Sums{};
+                    }
+                }
+            }
+        }
+        if (lane >= k0 && lane < k0 + cnt) my_f = base + (lane - k0);
+        k0 += cnt;
+        ra += cnt;
+        if (ra >= rb) take_chunk();
+        const bool more = ra < rb;
+        if (k0 == kWave || !more) {
+            // the group is full (or the last): its first loads, then its finish
+            const int ncnt = (int)min((long long)kWave, rb - ra);
+            if (kEarly && more)
+                prime_ring<kAxis, kGather, kRL, kRefRing>(
+                    xyz, ra, ncnt, frame_stride, rowstride, ix, n_sel, rx, ry, rz, n_blk, row, t,
+                    bx, by, bz, qx, qy, qz);
+            finish_group_at<kKind>(my_f, k0, n, rs, self_frame, out, rot_out, mu_out, s_res,
+                                   lane);
+            if (!kEarly && more)
+                prime_ring<kAxis, kGather, kRL, kRefRing>(
+                    xyz, ra, ncnt, frame_stride, rowstride, ix, n_sel, rx, ry, rz, n_blk, row, t,
+                    bx, by, bz, qx, qy, qz);
+            k0 = 0;
+        } else {
+            prime_ring<kAxis, kGather, kRL, kRefRing>(
+                xyz, ra, (int)min((long long)(kWave - k0), rb - ra), frame_stride, rowstride, ix,
+                n_sel, rx, ry, rz, n_blk, row, t, bx, by, bz, qx, qy, qz);
+        }
     }
 }
 

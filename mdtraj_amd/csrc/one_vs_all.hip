@@ -7,19 +7,24 @@
 // solve then runs one lane per frame (lane k finishes the k-th frame of the
 // wave's current group of 64).  The reference selection and the atom indices
 // are staged once per workgroup in LDS.
+#include <algorithm>
+
 #include "frames.h"
 
 namespace hr {
 namespace {
 
-// Per-frame kernel.  Each wave owns a contiguous chunk of frames and walks
-// its items (frame set, atom block) through a ring of kRing register
-// buffers, so the loads of the next kRing - 1 items are in flight while one
-// is accumulated in FP64.  A frame set is kWave / kRL frames, one per row of
-// kRL lanes.  At the end of a set each row's sums are reduced across its
-// lanes (DPP within 16 lanes, shuffles across) and parked in LDS; every 64
-// frames the QCP solve runs one lane per frame.  The grid is one workgroup
-// per resident slot (occupancy API), so no block waits for a second round.
+// Per-frame kernel.  One workgroup per CU holds all of the CU's waves
+// (kMinWaves per SIMD).  Each wave walks its items (frame set, atom block)
+// through a ring of kRing register buffers, so the loads of the next
+// kRing - 1 items are in flight while one is accumulated in FP64.  A frame
+// set is kWave / kRL frames, one per row of kRL lanes.  At the end of a set
+// each row's sums are reduced across its lanes (DPP within 16 lanes,
+// shuffles across) and parked in LDS; every 64 frames the QCP solve runs one
+// lane per frame.  The workgroup owns 1/G of the frame sets: each wave takes
+// a static share of the first static_pct percent of them, and the rest goes
+// out in chunks of `chunk` frames by an LDS ticket, so that the waves the
+// SIMD issues less often take fewer frames (process_cu).
 //   kAxis   target is axis-major (F, 3, rowstride) instead of atom-major
 //   kGather atom-major target with an index list
 //   kStage  what the workgroup stages in LDS: kStageAll the reference
@@ -27,30 +32,33 @@ namespace {
 //           gathered selection whose reference does not fit; the reference
 //           then rides the load ring); kStageNone nothing
 template <Kind kKind, bool kAxis, bool kGather, int kStage, int kMinWaves, int kRL>
-__global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
+__global__ __launch_bounds__(CuGroup<kMinWaves>::kThreads, kMinWaves) void frames_kernel(
     const float* __restrict__ xyz, long long n_frames, long long frame_stride, int rowstride,
     const int* __restrict__ idx, int n_sel, const float* __restrict__ ref_sel,
     const double* __restrict__ ref_stats, long long self_frame, float* __restrict__ out,
-    float* __restrict__ rot_out, double* __restrict__ mu_out) {
+    float* __restrict__ rot_out, double* __restrict__ mu_out, int static_pct, long long chunk) {
     constexpr int kNS = SumsPerFrame<kKind>::value;
+    constexpr int kWaves = CuGroup<kMinWaves>::kWaves;
+    constexpr int kThreads = CuGroup<kMinWaves>::kThreads;
     extern __shared__ double smem_d[];
+    __shared__ unsigned s_ticket;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     double* s_res = smem_d + wv * kWave * kNS;
-    float* smem = reinterpret_cast<float*>(smem_d + kWavesPerBlock * kWave * kNS);
+    float* smem = reinterpret_cast<float*>(smem_d + kWaves * kWave * kNS);
     const float* rx = ref_sel;
     const float* ry = ref_sel + n_sel;
     const float* rz = ref_sel + 2 * n_sel;
     const int* ix = idx;
+    if (threadIdx.x == 0) s_ticket = 0u;
     if constexpr (kStage != kStageNone) {
         // LDS after the result area: [reference planes (kStageAll)] [indices]
         float* s_rx = smem;
         float* s_ry = smem + n_sel;
         float* s_rz = smem + 2 * n_sel;
         int* s_idx = reinterpret_cast<int*>(kStage == kStageAll ? smem + 3 * n_sel : smem);
-        if constexpr (kStage == kStageAll) stage_words(s_rx, ref_sel, 3 * n_sel);
-        if constexpr (kGather) stage_words(s_idx, idx, n_sel);
-        __syncthreads();
+        if constexpr (kStage == kStageAll) stage_words<kThreads>(s_rx, ref_sel, 3 * n_sel);
+        if constexpr (kGather) stage_words<kThreads>(s_idx, idx, n_sel);
         if constexpr (kStage == kStageAll) {
             rx = s_rx;
             ry = s_ry;
@@ -58,6 +66,7 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
         }
         if constexpr (kGather) ix = s_idx;
     }
+    __syncthreads();
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
     if constexpr (kKind != Kind::NoSuperpose) {
         rs[0] = ref_stats[0];
@@ -65,17 +74,20 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void frames_kernel(
         rs[2] = ref_stats[2];
         rs[3] = ref_stats[3];
     }
-    // whole frame sets per wave (a set fills the wave's kWave / kRL rows)
+    // whole frame sets (a set fills the wave's kWave / kRL rows): the
+    // workgroup's sets b0 .. b1, the first n_static of them split evenly
+    // over its waves, the rest ticketed
     constexpr int kRows = kWave / kRL;
-    const long long wave = (long long)blockIdx.x * kWavesPerBlock + wv;
-    const long long n_waves = (long long)gridDim.x * kWavesPerBlock;
     const long long n_sets = (n_frames + kRows - 1) / kRows;
-    const long long f_begin = kRows * (wave * n_sets / n_waves);
-    const long long f_end = min(n_frames, kRows * ((wave + 1) * n_sets / n_waves));
-
-    process_chunk<kKind, kAxis, kGather, kRL, kStage != kStageAll>(
-        xyz, f_begin, f_end, frame_stride, rowstride, ix, n_sel, rx, ry, rz, rs, self_frame, out,
-        rot_out, mu_out, s_res, lane);
+    const long long g = blockIdx.x, n_groups = gridDim.x;
+    const long long b0 = g * n_sets / n_groups, b1 = (g + 1) * n_sets / n_groups;
+    const long long n_static = (b1 - b0) * static_pct / 100;
+    const long long s0 = b0 + wv * n_static / kWaves, s1 = b0 + (wv + 1) * n_static / kWaves;
+    process_cu<kKind, kAxis, kGather, kRL, kStage != kStageAll>(
+        xyz, min(n_frames, kRows * s0), min(n_frames, kRows * s1),
+        min(n_frames, kRows * (b0 + n_static)), min(n_frames, kRows * b1), chunk, &s_ticket,
+        frame_stride, rowstride, ix, n_sel, rx, ry, rz, rs, self_frame, out, rot_out, mu_out, s_res,
+        lane);
 }
 
 // In-place centering and traces, one wave per frame (two passes over the
@@ -153,18 +165,22 @@ struct Launch {
     int stage;  // kStageAll, kStageIdx or kStageNone
 };
 
-// LDS plan: the result area always; the reference selection and the indices
-// when they fit in 64 KB together with it, else the indices alone (gathered
-// selections), else nothing.  64 KB keeps three workgroups (12 waves) per CU
-// for selections up to ~2,400 contiguous or ~1,900 gathered atoms.
+// Waves per SIMD of the per-frame kernel (HR_TUNE_FRAMES_WAVES_PER_SIMD).
+int frames_waves_per_simd() { return tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4 ? 4 : 3; }
+
+// LDS plan of the one workgroup per CU: the result area of its waves
+// always; the reference selection and the indices when they fit in the CU's
+// LDS together with it, else the indices alone (gathered selections), else
+// nothing.  At 3 waves/SIMD (80 KB of results) that stages selections of up
+// to ~6,900 contiguous or ~5,200 gathered atoms.
 template <Kind kKind>
 int plan(int n_sel, bool gather, Launch* l) {
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
-    const size_t res = result_lds_bytes<kKind>();
+    const size_t res = result_lds_bytes<kKind>(4 * frames_waves_per_simd());
     const size_t ref = (size_t)n_sel * 3 * sizeof(float);
     const size_t ids = gather ? (size_t)n_sel * sizeof(int) : 0;
-    const size_t cap = d->lds_per_block < 65536 ? d->lds_per_block : 65536;
+    const size_t cap = std::min(d->lds_per_block, d->lds_per_cu) - 256;  // 256: static LDS
     if (res + ref + ids <= cap) {
         l->stage = kStageAll;
         l->lds = res + ref + ids;
@@ -178,38 +194,22 @@ int plan(int n_sel, bool gather, Launch* l) {
     return HR_OK;
 }
 
-// One workgroup per resident slot (occupancy API), capped by the work.
+// One workgroup per CU (occupancy API: the workgroup holds all of a CU's
+// waves), capped by the work: no more waves than frame sets.
 template <typename Kernel>
-int resident_blocks(Kernel kernel, size_t lds, long long n_waves_wanted, int* blocks) {
+int frames_grid(Kernel kernel, int threads, size_t lds, long long n_frames, int rows,
+                int* blocks) {
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
     int per_cu = 0;
     HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(kernel), kBlock, lds));
+        &per_cu, reinterpret_cast<const void*>(kernel), threads, lds));
     if (per_cu < 1) per_cu = 1;
     long long b = (long long)per_cu * d->cu_count;
-    const long long by_work = (n_waves_wanted + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (b > by_work) b = by_work;
-    *blocks = (int)(b < 1 ? 1 : b);
-    return HR_OK;
-}
-
-// Grid of the per-frame kernel; the kernel gives each wave whole frame sets
-// (`rows` frames, one per row).  Default: one wave per resident slot.
-// HR_TUNE_FRAMES_GRID = 1: with Q sets and S slots, k = ceil(Q / S) sets per
-// wave needs only ceil(Q / k) waves, and launching that many gives every wave
-// k or k - 1 sets.  On MI355X the full grid is faster (100,000 frames x 2,000
-// atoms, four frames per wave: 0.432 ms against 0.449 ms): more waves keep
-// more loads in flight, which outweighs the uneven last set.
-template <typename Kernel>
-int frames_grid(Kernel kernel, size_t lds, long long n_frames, int rows, int* blocks) {
     const long long n_sets = (n_frames + rows - 1) / rows;
-    int rc = resident_blocks(kernel, lds, n_sets, blocks);
-    if (rc || tuning(HR_TUNE_FRAMES_GRID) != 1) return rc;
-    const long long slots = (long long)*blocks * kWavesPerBlock;
-    const long long per_wave = (n_sets + slots - 1) / slots;
-    const long long waves = (n_sets + per_wave - 1) / per_wave;
-    const long long b = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int waves = threads / kWave;
+    const long long by_work = (n_sets + waves - 1) / waves;
+    if (b > by_work) b = by_work;
     *blocks = (int)(b < 1 ? 1 : b);
     return HR_OK;
 }
@@ -221,10 +221,29 @@ int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long 
                     float* out, float* rot_out, double* mu_out) {
     int blocks = 0, rc;
     auto k = frames_kernel<kKind, kAxis, kGather, kStage, kMinWaves, kRL>;
-    if ((rc = frames_grid(k, l.lds, n_frames, kWave / kRL, &blocks))) return rc;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), l.lds, s, xyz, n_frames, frame_stride,
+    constexpr int kThreads = CuGroup<kMinWaves>::kThreads;
+    constexpr int kRows = kWave / kRL;
+    static bool lds_attr_set = false;  // dynamic LDS beyond 64 KB needs the attribute
+    if (!lds_attr_set) {
+        const DeviceInfo* d = current_device();
+        if (!d) return HR_ERR_HIP;
+        HR_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(std::min(d->lds_per_block, d->lds_per_cu) - 256)));
+        lds_attr_set = true;
+    }
+    if ((rc = frames_grid(k, kThreads, l.lds, n_frames, kRows, &blocks))) return rc;
+    const int pct = tuning(HR_TUNE_FRAMES_STATIC_PCT) ? tuning(HR_TUNE_FRAMES_STATIC_PCT)
+                                                      : kDefaultStaticPct;
+    const long long kib = tuning(HR_TUNE_FRAMES_CHUNK_KIB) ? tuning(HR_TUNE_FRAMES_CHUNK_KIB)
+                                                          : kDefaultChunkKiB;
+    const long long frame_bytes = 12LL * (n_sel > 0 ? n_sel : 1);
+    long long chunk = (kib * 1024 + frame_bytes - 1) / frame_bytes;
+    chunk = std::max<long long>(kRows, std::min<long long>((chunk + kRows - 1) / kRows * kRows,
+                                                           kWave));
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), l.lds, s, xyz, n_frames, frame_stride,
                        rowstride, idx, n_sel, ref_sel, ref_stats, self_frame, out, rot_out,
-                       mu_out);
+                       mu_out, pct, chunk);
     return launch_status("frames_kernel");
 }
 
@@ -237,7 +256,7 @@ int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_
                   long long frame_stride, int rowstride, const int* idx, int n_sel,
                   const float* ref_sel, const double* ref_stats, long long self_frame, float* out,
                   float* rot_out, double* mu_out) {
-    const bool four = tuning(HR_TUNE_FRAMES_WAVES_PER_SIMD) == 4;
+    const bool four = frames_waves_per_simd() == 4;
     const int rl = tuning(HR_TUNE_FRAMES_ROW_LANES) ? tuning(HR_TUNE_FRAMES_ROW_LANES)
                                                    : auto_row_lanes(n_sel, kGather);
 #define HR_FRAMES_ARGS l, s, xyz, n_frames, frame_stride, rowstride, idx, n_sel, ref_sel, \

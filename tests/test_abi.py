@@ -58,7 +58,9 @@ def test_invalid_arguments_are_rejected_on_the_host():
         ("hr_rmsd_align_displace", (None, 3, 2, 4, None, None, 4, 4, None, None, None, None,
                                     None), "invalid sizes"),
         ("hr_batch_plan_create", (None, 0, None, ctypes.addressof(ws)), "no systems"),
-        ("hr_set_tuning", (7, 1), "unknown tuning knob"),
+        ("hr_set_tuning", (9, 1), "unknown tuning knob"),
+        ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_STATIC_PCT, 101), "must be 0 .. 100"),
+        ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_CHUNK_KIB, -1), "must be 0 or positive"),
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, 5), "must be 0, 3 or 4"),
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_GRID, 3), "must be 0, 1 or 2"),
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_ROW_LANES, 12), "must be 0, 4, 8, 16, 32 or 64"),
