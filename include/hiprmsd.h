@@ -71,7 +71,11 @@ int hr_device_cu_count(void);
  *                                  wave index and the rest from an atomic
  *                                  ticket (the tile shape applies to this
  *                                  one only); 2 the static split per wave,
- *                                  the reference read from global memory
+ *                                  the reference read from global memory;
+ *                                  4 as 3 with one workgroup per CU (up to
+ *                                  ~13,600 reference atoms in LDS) whose
+ *                                  waves share each system's frames through
+ *                                  an LDS counter
  *   HR_TUNE_SUPERPOSE_CHUNK_MIB    hr_superpose fits and rotates the frames
  *                                  in chunks of about this many MiB of
  *                                  coordinates (a fit pass, then a rotation

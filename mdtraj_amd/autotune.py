@@ -1,8 +1,9 @@
 """Autotuner for the batched plan's work split and tile shape.
 
 Candidates are (frames per work unit, KiB cap per unit, waves per SIMD of
-the scoring kernel, schedule).  Schedule 3 (one cost-balanced range per
-workgroup, the reference in LDS) has no unit shape.  Under schedule 1
+the scoring kernel, schedule).  Schedules 3 and 4 (one cost-balanced range
+per workgroup, the reference in LDS; 4: one workgroup per CU whose waves
+share each system's frames through an LDS counter) have no unit shape.  Under schedule 1
 (work units from a ticket) every unit is scored by one wave, whose
 bandwidth is bounded by the loads it keeps in flight, so the batch ends with
 the largest units still running: small units (and a byte cap, which
@@ -22,7 +23,7 @@ import os
 
 from . import _device as dev
 
-CANDIDATES = ([(64, 0, 3, 3)]
+CANDIDATES = ([(64, 0, 3, 3), (64, 0, 3, 4)]
               + [(fpu, kib, 3, 1) for fpu in (8, 16, 32, 64) for kib in (0, 64, 192)]
               + [(16, 0, 4, 1), (8, 64, 4, 1)])
 _memory = {}

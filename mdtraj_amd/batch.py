@@ -23,8 +23,9 @@ class BatchRMSD:
 
     ``systems``: list of float32 (F_s, N_s, 3) arrays or CUDA tensors (moved
     to the device once).  ``schedule``: the work split (``HR_TUNE_BATCH_SCHEDULE``:
-    3 one cost-balanced range per workgroup with the reference in LDS, 1 work
-    units from a ticket, 2 one range per wave).  Work-unit shape of schedule
+    3 one cost-balanced range per workgroup with the reference in LDS, 4 the
+    same with one workgroup per CU whose waves share each system's frames, 1
+    work units from a ticket, 2 one range per wave).  Work-unit shape of schedule
     1: ``frames_per_unit`` (1..64) and ``unit_kib`` (if > 0, fewer frames per
     unit, a multiple of 4, where frames_per_unit frames would exceed that
     many KiB); ``waves``: 3 or 4 waves per SIMD of the scoring kernel.  None

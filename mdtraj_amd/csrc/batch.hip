@@ -310,6 +310,87 @@ __global__ __launch_bounds__(kBlock, kMinWaves) void batch_block_kernel(
     }
 }
 
+// Per-CU split (schedule 4): as the block split, with one workgroup per CU
+// holding all of its waves (kMinWaves per SIMD).  The workgroup stages each
+// system's reference in the CU's LDS (up to ~13,600 atoms) and its waves
+// share the system's frames: a static share of the first static_pct percent
+// of the frame sets each, then chunks by an LDS ticket, so that the waves the
+// SIMD issues less often take fewer frames (process_segment_cu).
+template <int kMinWaves>
+__global__ __launch_bounds__(CuGroup<kMinWaves>::kThreads, kMinWaves) void batch_cu_kernel(
+    const DevSystem* __restrict__ sys, const long long* __restrict__ prefix,
+    const int* __restrict__ first, int n_systems, long long total, int lds_atoms,
+    int static_pct, int chunk_kib) {
+    constexpr int kWaves = CuGroup<kMinWaves>::kWaves;
+    constexpr int kThreads = CuGroup<kMinWaves>::kThreads;
+    extern __shared__ float s_ref[];
+    __shared__ unsigned s_ticket;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long g = blockIdx.x, n_groups = gridDim.x;
+    const long long c0 = split_point(total, n_groups, g);
+    const long long c1 = split_point(total, n_groups, g + 1);
+    const int lo = first[g];
+    DevSystem d = sys[lo];
+    long long start = prefix[lo];
+    for (int si = lo; si < n_systems && start < c1; ++si) {
+        const int sn = min(si + 1, n_systems - 1);
+        const DevSystem d_next = sys[sn];
+        const long long start_next = si + 1 < n_systems ? prefix[sn] : c1;
+        long long fa, fb;
+        frames_in(start, d.n_frames, d.n_atoms, c0, c1, &fa, &fb);
+        if (fa >= fb) {
+            d = d_next;
+            start = start_next;
+            continue;
+        }
+        const int n = d.n_atoms;
+        const int m = min(n, lds_atoms);
+        auto stage = [&]() {
+            __syncthreads();  // the previous system's reference and ticket are no longer used
+            if (m > 0) {
+                stage_words<kThreads>(s_ref, d.ref_sel, m);
+                stage_words<kThreads>(s_ref + m, d.ref_sel + n, m);
+                stage_words<kThreads>(s_ref + 2 * m, d.ref_sel + 2 * n, m);
+            }
+            if (threadIdx.x == 0) s_ticket = 0u;
+            __syncthreads();
+        };
+        const GlobalDouble* st = global_ptr(d.ref_stats);
+        const double rs[4] = {st[0], st[1], st[2], st[3]};
+        const GlobalFloat* gx = global_ptr(d.ref_sel);
+#define HR_CU_SEGMENT(RL)                                                                       \
+    do {                                                                                        \
+        constexpr long long kRows = kWave / (RL);                                               \
+        const long long n_sets = (fb - fa + kRows - 1) / kRows;                                 \
+        const long long n_static = n_sets * static_pct / 100;                                   \
+        const long long sa = min(fb, fa + kRows * (wv * n_static / kWaves));                    \
+        const long long sb = min(fb, fa + kRows * ((wv + 1) * n_static / kWaves));              \
+        const long long da = min(fb, fa + kRows * n_static);                                    \
+        long long ch = ((long long)chunk_kib * 1024 + 12LL * n - 1) / (12LL * n);               \
+        ch = max(kRows, min((long long)kWave, (ch + kRows - 1) / kRows * kRows));               \
+        if (m == n)                                                                             \
+            process_segment_cu<(RL), false>(d.xyz, sa, sb, da, fb, ch, &s_ticket, n, s_ref,    \
+                                            s_ref + m, s_ref + 2 * m, m, gx, gx + n,            \
+                                            gx + 2 * n, rs, (long long)d.ref_frame, d.out,      \
+                                            lane, stage);                                       \
+        else                                                                                    \
+            process_segment_cu<(RL), true>(d.xyz, sa, sb, da, fb, ch, &s_ticket, n, s_ref,     \
+                                           s_ref + m, s_ref + 2 * m, m, gx, gx + n, gx + 2 * n, \
+                                           rs, (long long)d.ref_frame, d.out, lane, stage);    \
+    } while (0)
+        switch (auto_row_lanes(n, false)) {
+            case 4: HR_CU_SEGMENT(4); break;
+            case 8: HR_CU_SEGMENT(8); break;
+            case 16: HR_CU_SEGMENT(16); break;
+            default: HR_CU_SEGMENT(32); break;
+        }
+#undef HR_CU_SEGMENT
+        d = d_next;
+        start = start_next;
+    }
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 void release(hr_batch_plan* p) {
@@ -331,12 +412,25 @@ int capture_plan(hr_batch_plan* p) {
     const DeviceInfo* d = current_device();
     if (!d) return HR_ERR_HIP;
     const int sched = p->schedule;
-    const void* k = sched == 3   ? reinterpret_cast<const void*>(batch_block_kernel<kMinWaves>)
+    const void* k = sched == 4   ? reinterpret_cast<const void*>(batch_cu_kernel<kMinWaves>)
+                    : sched == 3 ? reinterpret_cast<const void*>(batch_block_kernel<kMinWaves>)
                     : sched == 2 ? reinterpret_cast<const void*>(batch_static_kernel<kMinWaves>)
                                  : reinterpret_cast<const void*>(batch_kernel<kMinWaves>);
+    const int threads = sched == 4 ? CuGroup<kMinWaves>::kThreads : kBlock;
     size_t lds = (size_t)kWavesPerBlock * kWave * SumsPerFrame<Kind::Rmsd>::value *
                  sizeof(double);
-    if (sched == 3) {
+    if (sched == 4) {
+        // one workgroup per CU: the CU's LDS (less the static ticket)
+        const size_t cap = std::min(d->lds_per_cu, d->lds_per_block) / 1024 * 1024 - 1024;
+        p->lds_atoms = (int)std::min<long long>(p->max_atoms, (long long)cap / 12 / 4 * 4);
+        lds = (size_t)12 * p->lds_atoms;
+        static bool attr_set = false;
+        if (!attr_set) {
+            HR_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)cap));
+            attr_set = true;
+        }
+    } else if (sched == 3) {
         // the largest reference, up to what leaves room for kMinWaves
         // workgroups per CU.  A workgroup's LDS is allocated in granules:
         // 54,576 B passes the occupancy query for three per CU on MI355X,
@@ -355,16 +449,16 @@ int capture_plan(hr_batch_plan* p) {
         }
     }
     int per_cu = 0;
-    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds));
     long long blocks = (long long)(per_cu < 1 ? 1 : per_cu) * d->cu_count;
     // ticket: no more waves than units; static splits: no more than frames
     const long long work = sched == 1 ? p->n_units : p->total_frames;
-    const long long by_work = (work + kWavesPerBlock - 1) / kWavesPerBlock;
+    const long long by_work = (work + threads / kWave - 1) / (threads / kWave);
     if (blocks > by_work) blocks = by_work;
     if (blocks < 1) blocks = 1;
     p->blocks = (int)blocks;
     p->lds = lds;
-    if (sched == 3) {
+    if (sched == 3 || sched == 4) {
         // each workgroup's first system: the last one starting at or before
         // the workgroup's first cost
         std::vector<int> first(p->blocks);
@@ -383,7 +477,15 @@ int capture_plan(hr_batch_plan* p) {
     if (sched == 1) hipLaunchKernelGGL(ticket_reset_kernel, dim3(1), dim3(64), 0, s, p->d_ticket);
     hipLaunchKernelGGL(batch_ref_prep_kernel, dim3(p->n_systems), dim3(kBlock), 0, s,
                        static_cast<const DevSystem*>(p->d_systems));
-    if (sched == 3)
+    if (sched == 4)
+        hipLaunchKernelGGL(batch_cu_kernel<kMinWaves>, dim3(p->blocks), dim3(threads), lds, s,
+                           static_cast<const DevSystem*>(p->d_systems), p->d_prefix, p->d_first,
+                           p->n_systems, p->total_cost, p->lds_atoms,
+                           tuning(HR_TUNE_FRAMES_STATIC_PCT) ? tuning(HR_TUNE_FRAMES_STATIC_PCT)
+                                                             : kDefaultStaticPct,
+                           tuning(HR_TUNE_FRAMES_CHUNK_KIB) ? tuning(HR_TUNE_FRAMES_CHUNK_KIB)
+                                                            : kDefaultChunkKiB);
+    else if (sched == 3)
         hipLaunchKernelGGL(batch_block_kernel<kMinWaves>, dim3(p->blocks), dim3(kBlock), lds, s,
                            static_cast<const DevSystem*>(p->d_systems), p->d_prefix, p->d_first,
                            p->n_systems, p->total_cost, p->lds_atoms);

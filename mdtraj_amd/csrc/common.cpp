@@ -89,8 +89,8 @@ int hr_set_tuning(int knob, int value) {
         hr::set_error("HR_TUNE_FRAMES_ROW_LANES must be 0, 4, 8, 16, 32 or 64 (got %d)", value);
         return HR_ERR_INVALID;
     }
-    if (knob == HR_TUNE_BATCH_SCHEDULE && (value < 0 || value > 3)) {
-        hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0 .. 3 (got %d)", value);
+    if (knob == HR_TUNE_BATCH_SCHEDULE && (value < 0 || value > 4)) {
+        hr::set_error("HR_TUNE_BATCH_SCHEDULE must be 0 .. 4 (got %d)", value);
         return HR_ERR_INVALID;
     }
     if (knob == HR_TUNE_PAIRS_TILE && (value < 0 || value > 5)) {

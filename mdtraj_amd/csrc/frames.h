@@ -811,6 +811,140 @@ __device__ __forceinline__ void process_chunk_regs(
     }
 }
 
+// The batched per-CU workgroup's share of one system: process_chunk_regs's
+// branch-free item loop and register sums, with the frames [sa, sb) first
+// and then chunks of `chunk` frames from [da, db) by the LDS ticket
+// *s_ticket, shared by the workgroup's waves (so that the waves the SIMD
+// issues less often take fewer frames).  Chunks fill the wave's group of 64
+// frame slots in turn, lane k keeping the index of the frame in slot k.
+template <int kRL, bool kPartial, typename Stage>
+__device__ __forceinline__ void process_segment_cu(
+    const float* __restrict__ xyz, long long sa, long long sb, long long da, long long db,
+    long long chunk, unsigned* s_ticket, int n_sel, const float* __restrict__ lx,
+    const float* __restrict__ ly, const float* __restrict__ lz, int n_lds,
+    const GlobalFloat* __restrict__ gx, const GlobalFloat* __restrict__ gy,
+    const GlobalFloat* __restrict__ gz, const double (&rs)[4], long long self_frame,
+    float* __restrict__ out, int lane, Stage&& stage) {
+    constexpr int kRows = kWave / kRL;
+    const int row = lane / kRL, t = lane % kRL;
+    const long long frame_stride = 3LL * n_sel;
+    const double n = (double)n_sel;
+    const int n_blk = (n_sel + kRL * kItemAtoms - 1) / (kRL * kItemAtoms);
+    float bx[kRing][kItemAtoms], by[kRing][kItemAtoms], bz[kRing][kItemAtoms];
+    long long ra = sa, rb = sb;
+    auto take_chunk = [&]() {
+        unsigned c = 0;
+        if (lane == 0) c = atomicAdd(s_ticket, 1u);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
+        ra = da + (long long)c * chunk;
+        rb = ra < db ? min(db, ra + chunk) : ra;
+    };
+    // the static range's first loads go out before the caller stages the
+    // reference (barriers included; it also clears the ticket)
+    if (ra < rb)
+        prime_ring_regs<kRL>(xyz, ra, (int)min((long long)kWave, rb - ra), n_sel, n_blk, row, t,
+                             bx, by, bz);
+    stage();
+    if (ra >= rb) {
+        take_chunk();
+        if (ra >= rb) return;
+        prime_ring_regs<kRL>(xyz, ra, (int)min((long long)kWave, rb - ra), n_sel, n_blk, row, t,
+                             bx, by, bz);
+    }
+    int k0 = 0;          // slots of the group filled so far
+    long long my_f = 0;  // frame of this lane's slot
+    Sums mine = {};
+    while (ra < rb) {
+        const long long base = ra;
+        const int cnt = (int)min((long long)(kWave - k0), rb - ra);
+        const int n_items = (cnt + kRows - 1) / kRows * n_blk;
+        const int n_steps = (n_items + kRing - 1) / kRing * kRing;
+        // lanes k0 + kRows q .. take set q's sums from row (lane - k0) mod kRows
+        const int rel = lane - k0;
+        const int src = ((rel > 0 ? rel : 0) % kRows) * kRL;
+        Sums s = {};
+        for (int it0 = 0; it0 < n_steps; it0 += kRing) {
+#pragma unroll
+            for (int r = 0; r < kRing; ++r) {
+                const int it = it0 + r;
+                {
+                    const int nxt = min(it + kRing - 1, n_items - 1);
+                    const int q = nxt / n_blk, b = nxt - q * n_blk;
+                    const int k = kRows * q + row;
+                    const int slot = (r + kRing - 1) % kRing;
+                    load_item<false, false, kRL, true>(
+                        xyz + (base + (k < cnt ? k : 0)) * frame_stride, 0, nullptr, n_sel, t, b,
+                        bx[slot], by[slot], bz[slot]);
+                }
+                const bool live = it < n_items;  // wave-uniform
+                const int itc = live ? it : n_items - 1;
+                const int q = itc / n_blk, b = itc - q * n_blk;
+                const int j0 = b * kRL * kItemAtoms + t;
+                const bool row_ok = live && kRows * q + row < cnt;
+                float ux[kItemAtoms], uy[kItemAtoms], uz[kItemAtoms];
+                const bool from_lds = !kPartial || (b + 1) * kRL * kItemAtoms <= n_lds;
+                if (from_lds) {
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int jc = min(j0 + u * kRL, n_sel - 1);
+                        ux[u] = lx[jc];
+                        uy[u] = ly[jc];
+                        uz[u] = lz[jc];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kItemAtoms; ++u) {
+                        const int jc = min(j0 + u * kRL, n_sel - 1);
+                        ux[u] = gx[jc];
+                        uy[u] = gy[jc];
+                        uz[u] = gz[jc];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kItemAtoms; ++u) {
+                    const int j = j0 + u * kRL;
+                    const bool ok = row_ok && j < n_sel;
+                    accumulate<Kind::Rmsd>(s, valid_atom(ok, bx[r][u]), valid_atom(ok, by[r][u]),
+                                           valid_atom(ok, bz[r][u]), ux[u], uy[u], uz[u]);
+                }
+                if (live && b == n_blk - 1) {  // wave-uniform: the set's last block
+                    row_reduce<Kind::Rmsd, kRL>(s);
+                    Sums got;
+                    got.q = __shfl(s.q, src, kWave);
+                    got.sx = __shfl(s.sx, src, kWave);
+                    got.sy = __shfl(s.sy, src, kWave);
+                    got.sz = __shfl(s.sz, src, kWave);
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) got.p[e] = __shfl(s.p[e], src, kWave);
+                    take<Kind::Rmsd>(mine, got, rel >= kRows * q && rel < kRows * q + kRows);
+                    s = Sums{};
+                }
+            }
+        }
+        if (rel >= 0 && rel < cnt) my_f = base + rel;
+        k0 += cnt;
+        ra += cnt;
+        if (ra >= rb) take_chunk();
+        const bool more = ra < rb;
+        const bool full = k0 == kWave || !more;
+        // the next items' first loads before the QCP solves (unconditional:
+        // with no more work the last group is loaded again and not used)
+        const long long pb = more ? ra : base;
+        const int pc = more ? (int)min((long long)(full ? kWave : kWave - k0), rb - ra) : cnt;
+        prime_ring_regs<kRL>(xyz, pb, pc, n_sel, n_blk, row, t, bx, by, bz);
+        if (full) {
+            if (lane < k0) {
+                double M[9], ga, gb;
+                centred(mine, rs, n, M, ga, gb);
+                const double msd = qcp_msd(M, ga, gb, n);
+                global_out(out)[my_f] = (my_f == self_frame) ? 0.0f : (float)sqrt(msd);
+            }
+            k0 = 0;
+            mine = Sums{};
+        }
+    }
+}
+
 // Batched variant of the item loop (one system's frames, all atoms, no
 // gather).  Each wave works on its own system, so the reference is not
 // staged in LDS up front: every item's block of 64 reference atoms rides in

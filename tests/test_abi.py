@@ -64,7 +64,7 @@ def test_invalid_arguments_are_rejected_on_the_host():
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_WAVES_PER_SIMD, 5), "must be 0, 3 or 4"),
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_GRID, 3), "must be 0, 1 or 2"),
         ("hr_set_tuning", (_lib.HR_TUNE_FRAMES_ROW_LANES, 12), "must be 0, 4, 8, 16, 32 or 64"),
-        ("hr_set_tuning", (_lib.HR_TUNE_BATCH_SCHEDULE, 4), "must be 0 .. 3"),
+        ("hr_set_tuning", (_lib.HR_TUNE_BATCH_SCHEDULE, 5), "must be 0 .. 4"),
         ("hr_set_tuning", (_lib.HR_TUNE_SUPERPOSE_CHUNK_MIB, -1), "must be 0 or positive"),
         ("hr_centroid_scores", (ctypes.addressof(ws) + 4, 10, 1.0, ctypes.addressof(ws), None,
                                 ctypes.addressof(ws), None), "16-byte aligned"),

@@ -23,7 +23,7 @@ def systems(seed=0, n=24):
     return out
 
 
-@pytest.mark.parametrize("schedule", [1, 2, 3])
+@pytest.mark.parametrize("schedule", [1, 2, 3, 4])
 @pytest.mark.parametrize("fpu,kib,waves", [(64, 0, 3), (16, 0, 3), (7, 0, 4), (64, 0, 4),
                                            (64, 8, 3), (8, 64, 3)])
 def test_batch_vs_fp64(gpu, fpu, kib, waves, schedule):
@@ -48,17 +48,20 @@ def test_batch_vs_fp64(gpu, fpu, kib, waves, schedule):
     b.close()
 
 
+@pytest.mark.parametrize("schedule", [3, 4])
 @pytest.mark.parametrize("waves", [3, 4])
-def test_block_split_reference_beyond_lds(gpu, waves):
-    """The per-workgroup split stages references of up to ~4,500 atoms (3
-    waves/SIMD) or ~3,400 (4) in LDS; larger systems read theirs from global
-    memory.  Mixed in one plan, both paths match the FP64 oracle."""
+def test_block_split_reference_beyond_lds(gpu, waves, schedule):
+    """The per-workgroup split stages references of up to ~4,400 atoms (3
+    waves/SIMD) or ~3,400 (4) in LDS, the per-CU split up to ~13,600; the
+    rest of a larger reference is read from global memory.  Mixed in one
+    plan, both paths match the FP64 oracle."""
     xs = []
-    for s, (F, N) in enumerate([(9, 3500), (70, 40), (5, 5000), (33, 4400), (3, 9000), (65, 1)]):
+    for s, (F, N) in enumerate([(9, 3500), (70, 40), (5, 5000), (33, 4400), (3, 9000), (65, 1),
+                                (2, 15000), (130, 700)]):
         x, _ = gen.small_cube(seed=2000 + s, n_frames=F, n_atoms=N, sigma=0.03)
         xs.append(x)
     refs = [x.shape[0] // 2 for x in xs]
-    b = BatchRMSD(xs, refs, waves=waves)
+    b = BatchRMSD(xs, refs, frames_per_unit=64, unit_kib=0, waves=waves, schedule=schedule)
     res = [dev.to_host(r) for r in b.run()]
     for x, r, got in zip(xs, refs, res):
         want = fp64.rmsd(x, x, r)
