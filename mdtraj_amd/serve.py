@@ -10,10 +10,15 @@ allocation) fails every outstanding request: ``Request.wait`` raises it.
 
 Continuous batching: a scheduler thread takes the oldest pending request,
 gathers every other pending request for the same pool up to
-``max_batch_queries`` query frames, and launches the batch (upload, split
-prep, one cross-Gram pass over the pool, copy of the results to pinned host
-memory) on the next stream of a small ring as soon as a slot is free and the
-HBM budget admits its buffers.  Requests that arrive while earlier batches
+``max_batch_queries`` query frames (fewer on large pools, under
+``max_batch_work``), and launches the batch (upload from a pinned staging
+buffer, split prep, one cross-Gram pass over the pool, copy of the results
+to pinned host memory) on the next stream of a small ring as soon as a slot
+is free and the HBM budget admits its buffers.  ``start()`` allocates what
+the largest batch needs before serving (pinned buffers, device memory on
+each stream) and freezes the objects alive at that point out of Python's
+garbage collection: either, left to happen while serving, held the
+scheduler for tens of milliseconds.  Requests that arrive while earlier batches
 are on the GPU join later batches; batches on different streams overlap on
 the device.  A completion thread retires batches as their events complete,
 in whatever order that is, and hands every request its rows.  Each (query, pool frame) result depends only on those two
@@ -23,6 +28,7 @@ it was scored alone or batched with others.
 from __future__ import annotations
 
 import collections
+import gc
 import threading
 import time
 
@@ -81,17 +87,24 @@ class RMSDServer:
     max_batch_queries: query frames per batch; a batch reads its pool once
     whatever its size (one MFMA row tile per 64 query frames), so under load
     larger batches cost less per query.
+    max_batch_work: query frames x pool frames x atoms one batch may cover
+    beyond its first request (default 2.6e10, about 2 ms of split-FP16 MFMA
+    work on MI355X): a batch's requests wait for all of its rows, so batches
+    on the largest pools are kept short while those on small pools still
+    fill up to max_batch_queries.
     n_streams: batches that may be in flight at once.
     keep_results: copy every request's rows out of the reused pinned buffer
     into its own array (False: only the arrival-to-rows-on-host latency is
     recorded, for clients that consume the rows as they land)."""
 
     def __init__(self, hbm_budget=200e9, mode="f16exact", max_batch_queries=256,
-                 max_request_queries=64, n_streams=3, keep_results=True):
+                 max_request_queries=64, n_streams=4, keep_results=True,
+                 max_batch_work=2.6e10):
         self.t = dev.require_torch_gpu()
         self.mode = mode
         self.budget = int(hbm_budget)
         self.max_q = int(max_batch_queries)
+        self.max_work = float(max_batch_work)
         self.max_req_q = int(max_request_queries)
         if not 1 <= self.max_req_q <= self.max_q:
             raise ValueError("need 1 <= max_request_queries <= max_batch_queries")
@@ -105,6 +118,8 @@ class RMSDServer:
         # one pinned result buffer per stream slot, reused by every batch on
         # that slot (the slot is free again only after its rows were copied out)
         self.pinned = [None] * n_streams
+        # and one pinned staging buffer per slot for the batch's query frames
+        self.pinned_in = [None] * n_streams
         self.error = None
         self.pending = collections.deque()
         self.cv = threading.Condition()
@@ -158,8 +173,9 @@ class RMSDServer:
 
     def _take_batch(self, avail):
         """Oldest pending request plus later ones for the same pool (FIFO
-        within the pool), up to max_q query frames and `avail` bytes of HBM
-        in flight.  The oldest request must fit `avail`."""
+        within the pool), up to max_q query frames, max_work and `avail`
+        bytes of HBM in flight.  The oldest request is always taken (it must
+        fit `avail`)."""
         first = self.pending[0]
         pool = self.pools[first.pool]
         batch, nq = [], 0
@@ -168,7 +184,8 @@ class RMSDServer:
             r = self.pending.popleft()
             q = r.queries.shape[0]
             if (r.pool == first.pool and nq + q <= self.max_q
-                    and (not batch or self._batch_bytes(pool, nq + q) <= avail)):
+                    and (not batch or (nq + q <= self._max_batch_queries(pool)
+                                       and self._batch_bytes(pool, nq + q) <= avail))):
                 batch.append(r)
                 nq += q
             else:
@@ -223,12 +240,52 @@ class RMSDServer:
         except BaseException as exc:  # noqa: BLE001 - handed to every waiter
             self._fail_all(exc)
 
+    def _max_batch_queries(self, pool):
+        """Query frames a batch on `pool` may grow to past its first request:
+        max_q, or fewer under max_batch_work."""
+        by_work = int(self.max_work // (float(pool.n_frames) * pool.n_atoms))
+        return min(self.max_q, by_work)
+
+    def _largest_batch(self, pool):
+        """The most query frames a batch on `pool` can hold: its cap, or one
+        request of max_request_queries, which is always admitted."""
+        return max(self.max_req_q, self._max_batch_queries(pool))
+
+    def _reserve(self):
+        """Allocate, before serving, what the largest batch on any pool needs:
+        each stream slot's pinned result buffer, and device memory for the
+        batches in flight (kept by torch's caching allocator).  Allocated
+        lazily, a pinned buffer of ~190 MB for a slot's first batch on a
+        large pool held the scheduler for tens of ms, and every request that
+        arrived meanwhile queued behind it."""
+        if not self.pools:
+            return
+        t = self.t
+        rows = max(self._largest_batch(p) * p.n_frames for p in self.pools)
+        for si in range(len(self.pinned)):
+            if self.pinned[si] is None or self.pinned[si].numel() < rows:
+                self.pinned[si] = t.empty(rows, dtype=t.float32, pin_memory=True)
+        # device memory: torch's caching allocator keeps a freed block for
+        # the stream it was used on, so each slot reserves on its own stream
+        one = max(self._batch_bytes(p, self._largest_batch(p)) for p in self.pools)
+        room = max(0, self.budget - self.resident)
+        qbytes = max(12 * self._largest_batch(p) * p.n_atoms for p in self.pools)
+        for si, stream in enumerate(self.streams):
+            if self.pinned_in[si] is None or self.pinned_in[si].numel() < qbytes // 4:
+                self.pinned_in[si] = t.empty(qbytes // 4, dtype=t.float32, pin_memory=True)
+            if one <= room:
+                room -= one
+                with t.cuda.stream(stream), dev.torch_buffers():
+                    block = t.empty(one, dtype=t.uint8, device=dev.device())
+                    del block
+        t.cuda.synchronize()
+
     def _pinned_rows(self, si, nq, nf):
         """(nq, nf) view of stream slot si's pinned result buffer."""
         t = self.t
         need = nq * nf
         buf = self.pinned[si]
-        if buf is None or buf.numel() < need:
+        if buf is None or buf.numel() < need:  # a pool added after start()
             buf = t.empty(max(need, self.max_q * nf), dtype=t.float32, pin_memory=True)
             self.pinned[si] = buf
         return buf[:need].view(nq, nf)
@@ -236,10 +293,17 @@ class RMSDServer:
     def _launch(self, pool, batch, nq, si, need):
         t = self.t
         stream = self.streams[si]
-        queries = np.concatenate([r.queries for r in batch], axis=0)
+        n_q = sum(r.queries.shape[0] for r in batch)
+        staged = self.pinned_in[si]
+        if staged is None or staged.numel() < n_q * pool.n_atoms * 3:  # a pool added after start()
+            staged = t.empty(n_q * pool.n_atoms * 3, dtype=t.float32, pin_memory=True)
+            self.pinned_in[si] = staged
+        host_q = staged[: n_q * pool.n_atoms * 3].view(n_q, pool.n_atoms, 3)
+        np.concatenate([r.queries for r in batch], axis=0, out=host_q.numpy())
         stream.wait_event(pool.ready)
         with t.cuda.stream(stream), dev.torch_buffers():
-            dq = t.from_numpy(queries).to(device=dev.device())
+            # the slot's staging buffer is free: its previous batch completed
+            dq = host_q.to(device=dev.device(), non_blocking=True)
             qs = SplitSet(dq, mode=self.mode)
             out = rmsd_cross(qs, pool.split)
             host_out = self._pinned_rows(si, nq, pool.n_frames)
@@ -314,6 +378,12 @@ class RMSDServer:
 
     # ------------------------------------------------------------- lifecycle
     def start(self):
+        self._reserve()
+        # Python's oldest-generation collection walks every live object (with
+        # torch loaded, hundreds of thousands): 35-50 ms in which neither
+        # thread launches or retires a batch.  Objects that exist now (the
+        # process's modules, the pools) are moved out of later collections.
+        gc.freeze()
         self.running = True
         self._sched = threading.Thread(target=self._scheduler, daemon=True)
         self._comp = threading.Thread(target=self._completer, daemon=True)

@@ -26,6 +26,10 @@ class _Cuda:
     def stream(s):
         yield
 
+    @staticmethod
+    def synchronize():
+        pass
+
     class Event:
         def record(self, stream=None):
             pass
@@ -188,3 +192,34 @@ def test_budget_counts_one_request_batch(cpu_server):
     srv = cpu_server(hbm_budget=probe.pools[p].nbytes + 10)
     with pytest.raises(MemoryError, match="one 64-query batch"):
         srv.add_pool(pool)
+
+
+def test_batch_work_cap_keeps_large_pool_batches_short(cpu_server):
+    """A batch covers at most max_batch_work query x pool frame x atom
+    products beyond its first request: on a large pool requests are batched
+    few at a time, on a small pool up to max_batch_queries, and a request
+    larger than the cap still runs alone."""
+    big = gen.walk(seed=8, n_frames=200, n_atoms=30)     # 6,000 per query
+    small = gen.walk(seed=9, n_frames=20, n_atoms=10)    # 200 per query
+    srv = cpu_server(hbm_budget=1e9, max_batch_queries=128, max_batch_work=6000 * 20)
+    pb, ps = srv.add_pool(big), srv.add_pool(small)
+    reqs = [srv.submit(pb, big[i:i + 8]) for i in range(0, 80, 8)]      # 8 queries each
+    reqs += [srv.submit(ps, small[:16]) for _ in range(6)]              # 16 queries each
+    reqs.append(srv.submit(pb, big[:40]))                               # 40 > the cap alone
+    sizes = []
+    orig = srv._take_batch
+
+    def spy(avail):
+        pool, batch, nq = orig(avail)
+        sizes.append((pool is srv.pools[pb], nq))
+        return pool, batch, nq
+
+    srv._take_batch = spy
+    with srv:
+        for r in reqs:
+            r.wait(30)
+    big_sizes = [nq for is_big, nq in sizes if is_big]
+    small_sizes = [nq for is_big, nq in sizes if not is_big]
+    assert max(nq for nq in big_sizes if nq != 40) <= 16   # 2 requests of 8 = 96,000 work
+    assert 40 in big_sizes                                  # the oversized request, alone
+    assert max(small_sizes) == 96                           # all six small requests at once
