@@ -56,9 +56,10 @@ def traffic_from_profile(name, shape=None):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if shape is not None and "shape" in rec and list(rec["shape"]) != list(shape):
-        return None
-    return rec.get("hbm_bytes_per_launch")
+    for r in rec.get("records", [rec]):  # one record, or one per shape
+        if shape is None or "shape" not in r or list(r["shape"]) == list(shape):
+            return r.get("hbm_bytes_per_launch")
+    return None
 
 
 def _probe_rate(entry, x, nbytes, blocks, warmup, steps):
