@@ -861,20 +861,40 @@ def bench_nearest_leader(args):
                   full.data_ptr(), full.stride(0), dev.stream())
         torch.argmin(full, dim=1)
 
+    K = 10
+    tk_idx = torch.empty((F, K), dtype=torch.int32, device=X.device)
+    tk_val = torch.empty((F, K), dtype=torch.float32, device=X.device)
+
+    def nearest10():  # each frame's 10 nearest leaders: cross matrix, then hr_topk_rows
+        _lib.call("hr_rmsd_cross", a.block.data_ptr(), F, b.block.data_ptr(), L, N, mode,
+                  full.data_ptr(), full.stride(0), dev.stream())
+        _lib.call("hr_topk_rows", full.data_ptr(), F, L, full.stride(0), K, tk_idx.data_ptr(),
+                  tk_val.data_ptr(), dev.stream())
+
+    def topk_only():
+        _lib.call("hr_topk_rows", full.data_ptr(), F, L, full.stride(0), K, tk_idx.data_ptr(),
+                  tk_val.data_ptr(), dev.stream())
+
     ms = _time_events(fused, args.steps, args.warmup)
     ms_unfused = _time_events(unfused, args.steps, args.warmup)
+    ms_k10 = _time_events(nearest10, args.steps, args.warmup)
+    ms_topk = _time_events(topk_only, args.steps, args.warmup)
     unfused()
     assert torch.equal(idx.long(), torch.argmin(full, dim=1)), "fused argmin differs"
+    topk_only()
+    assert torch.equal(tk_idx[:, 0].long(), idx.long()), "top-10 nearest differs from the argmin"
     useful = 18.0 * N * F * L  # the nine-entry correlation of every pair
     tflops = useful / (ms * 1e-3) / 1e12
     log(f"nearest leader {F} x {L} leaders x {N} atoms: fused {ms:.3f} ms, unfused "
-        f"{ms_unfused:.3f} ms, {tflops:.1f} TFLOP/s useful")
+        f"{ms_unfused:.3f} ms, {tflops:.1f} TFLOP/s useful; 10 nearest {ms_k10:.3f} ms "
+        f"(top-k pass {ms_topk:.3f} ms)")
     return {"metric": "frames assigned to their nearest leader per second", "value": F / (ms * 1e-3),
             "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": args.mode, "data": "synthetic (seeded random walk generated in HBM)",
             "config": {"workload": f"nearest of {L} leaders for {F} frames x {N} atoms, mode {args.mode}",
-                       "unfused_ms": ms_unfused, "speedup_vs_unfused": ms_unfused / ms},
+                       "unfused_ms": ms_unfused, "speedup_vs_unfused": ms_unfused / ms,
+                       "nearest10_ms": ms_k10, "topk10_pass_ms": ms_topk},
             "roofline": {"bound": "mfma", "achieved": tflops, "peak": F16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": tflops / F16_PEAK_TFLOPS, "traffic": None}}
 

@@ -343,6 +343,17 @@ size_t hr_centroid_workspace_bytes(int64_t n_frames);
 int hr_centroid_scores(const float* condensed, int64_t n_frames, double beta, double* scores,
                        double* stats, void* workspace, void* stream);
 
+/* hr_topk_rows: the k nearest columns of every row of a float32 matrix (an
+ * hr_rmsd_cross output: frames x leaders), generalising the argmin of the
+ * reference's leader assignment (examples/two-pass-clustering.ipynb, cell 6,
+ * `np.argmin(md.rmsd(leaders, frame, 0))`) to its k smallest entries:
+ * index[i k + t], value[i k + t] for t = 0 .. k-1 in ascending order, ties to
+ * the lower column, NaN last -- np.argsort(row, kind="stable")[:k].  Device
+ * pointers; row i starts at matrix + i ld; 1 <= k <= n_cols < 2^31.  No
+ * allocation. */
+int hr_topk_rows(const float* matrix, int64_t n_rows, int64_t n_cols, int64_t ld, int32_t k,
+                 int32_t* index, float* value, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Batched one-vs-all over many independent systems of uneven size (no
  * counterpart in the reference, whose users loop md.rmsd per trajectory).

@@ -93,6 +93,7 @@ _SIGNATURES = {
     "hr_rmsd_cross_argmin": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp]),
     "hr_centroid_workspace_bytes": (_sz, [_i64]),
     "hr_centroid_scores": (ctypes.c_int, [_vp, _i64, ctypes.c_double, _vp, _vp, _vp, _vp]),
+    "hr_topk_rows": (ctypes.c_int, [_vp, _i64, _i64, _i64, ctypes.c_int32, _vp, _vp, _vp]),
     "hr_xtc_decode": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hr_dcd_scan": (ctypes.c_int, [_vp, _i64, _vp, _vp, _i32]),
     "hr_lprmsd_workspace_bytes": (_sz, [_i32, _i32]),

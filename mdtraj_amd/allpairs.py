@@ -129,6 +129,37 @@ def nearest(a, b):
     return index, value
 
 
+def topk_rows(matrix, k):
+    """The k smallest entries of every row of a float32 device matrix (an
+    ``rmsd_cross`` output), ascending, ties to the lower column, NaN last
+    (``np.argsort(row, kind="stable")[:k]``): (index int32 (rows, k), value
+    float32 (rows, k)) device arrays (``hr_topk_rows``)."""
+    k = int(k)
+    with dev.using(matrix):
+        n_rows, n_cols = int(matrix.shape[0]), int(matrix.shape[1])
+        if not 1 <= k <= n_cols:
+            raise ValueError(f"k must be in [1, {n_cols}]")
+        index = dev.empty((n_rows, k), np.int32)
+        value = dev.empty((n_rows, k), np.float32)
+        call("hr_topk_rows", matrix.data_ptr(), n_rows, n_cols, matrix.stride(0), k,
+             index.data_ptr(), value.data_ptr(), dev.stream())
+    return index, value
+
+
+def nearest_k(a, b, k):
+    """For every frame of prepared set ``a``, its k nearest frames of set
+    ``b``, nearest first (ties to the lower index): (index, RMSD) device
+    arrays (a.n_frames, k).  The k-leader generalisation of ``nearest``:
+    the cross matrix (4 a.n_frames b.n_frames bytes of HBM, freed on
+    return), then ``topk_rows``."""
+    if a.mode != b.mode or a.n_sel != b.n_sel:
+        raise ValueError("sets must share the mode and the number of selected atoms")
+    if not 1 <= int(k) <= b.n_frames:
+        raise ValueError(f"k must be in [1, {b.n_frames}]")
+    d = rmsd_cross(a, b)
+    return topk_rows(d, k)
+
+
 def assign_to_leaders(frames, leaders, atom_indices=None, mode="f16exact"):
     """Index of the nearest leader for every frame, and that RMSD (numpy):
     ``argmin(md.rmsd(leaders, frame, 0))`` for all frames in one pass."""
