@@ -357,14 +357,20 @@ int hr_topk_rows(const float* matrix, int64_t n_rows, int64_t n_cols, int64_t ld
 /* ---------------------------------------------------------------------------
  * Batched one-vs-all over many independent systems of uneven size (no
  * counterpart in the reference, whose users loop md.rmsd per trajectory).
- * A plan captures, once, a hipGraph of three kernels (ticket reset, reference
- * prep, persistent scoring kernel whose waves pull (system, frame chunk)
- * units, largest first, from an atomic ticket); each launch is one graph
- * launch.  Plan creation allocates and copies the descriptor and unit tables;
- * launches allocate nothing.  The systems' buffers must stay valid for the
- * plan's lifetime.  Do not launch one plan on two streams at once.
- *   shape            work-unit shape (NULL = defaults); the autotuner in
- *                    mdtraj_amd/autotune.py picks it per workload:
+ * A plan captures, once, a hipGraph: reference prep (one workgroup per
+ * system), then the scoring kernel of the work split HR_TUNE_BATCH_SCHEDULE
+ * selects when the plan is created (default: one cost-balanced range of all
+ * systems' frames per workgroup, each system's reference staged in LDS; the
+ * ticket schedule adds a ticket reset kernel first and has its waves pull
+ * (system, frame chunk) units, largest first, from an atomic ticket).  Each
+ * launch is one graph launch.  Plan creation allocates and copies the
+ * descriptor, unit and range tables; launches allocate nothing.  The
+ * systems' buffers must stay valid for the plan's lifetime.  Do not launch
+ * one plan on two streams at once.
+ *   shape            work-unit shape of the ticket schedule (NULL =
+ *                    defaults; waves_per_simd applies to every schedule);
+ *                    the autotuner in mdtraj_amd/autotune.py picks the
+ *                    schedule and shape per workload:
  *     frames_per_unit  frames per work unit, 1..64 (0 = 64)
  *     unit_kib         if > 0, a system's units hold fewer frames (a multiple
  *                      of 4, at least 4) when frames_per_unit frames would
