@@ -688,6 +688,12 @@ __global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramA
     const char* src[kHMaxLoads];
     plan_loads_half<Half, kS>(g, i0, j0, n_chunks, first, lane, src);
     issue_stage_half<Half>(src, first, count, 0, lds);
+    // The two workgroups of a CU drift out of phase, so one wave of a SIMD can
+    // be in the Gram loop while the other solves its FP64 epilogue: the loop
+    // runs at raised issue priority so that its MFMAs are issued first and
+    // the epilogue fills the gaps (measured 0.06-0.1 ms per 20,000-frame
+    // matrix; the results are the same bits).
+    __builtin_amdgcn_s_setprio(1);
     for (int c = 0; c < n_chunks; ++c) {
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's chunk-c loads landed
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): its chunk c-1 LDS reads are done
@@ -736,6 +742,7 @@ __global__ __launch_bounds__(kHBlock, 2) void pairs_mfma_half_kernel(const GramA
             }
         }
     }
+    __builtin_amdgcn_s_setprio(0);
     if constexpr (kRegEpi) {
         // Lane l holds, for rb and v, the nine entries of pair (row
         // 32 wi + 16 rb + 4 (l / 16) + v, column 16 wj + l % 16): its QCP runs

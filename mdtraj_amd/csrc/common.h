@@ -185,9 +185,12 @@ __host__ __device__ __forceinline__ NewtonState newton_start(const Quartic& q, d
 
 // One Newton step, branch-free (so that two solves can be interleaved): the
 // new point is taken only if it is still verified above the roots; the
-// iteration stops once a step falls below 1e-8 x -- quadratic convergence
-// then leaves the final estimate x - P/P' about 1e-16 x from the root, which
-// the bracket of newton_finish certifies (or not, at a near-double root).
+// iteration stops once a step falls below 1e-4 x -- quadratic convergence
+// then leaves x about 1e-8 x above a simple root and the final estimate
+// x - P/P' about 1e-16 x from it, which the bracket of newton_finish
+// certifies (or not: a near-double root converges slowly, fails the
+// bracket and takes the Jacobi fallback).  A 1e-8 stop costs one more
+// step on nearly every solve for no accuracy on a certified one.
 __host__ __device__ __forceinline__ void newton_step(const Quartic& q, NewtonState& s) {
     const double nx = s.x - s.p * recip(s.d1);
     double np, nd1, nd2;
@@ -198,7 +201,7 @@ __host__ __device__ __forceinline__ void newton_step(const Quartic& q, NewtonSta
     s.p = take ? np : s.p;
     s.d1 = take ? nd1 : s.d1;
     s.d2 = take ? nd2 : s.d2;
-    s.run = take && step > 1e-8 * nx;
+    s.run = take && step > 1e-4 * nx;
 }
 
 // Final estimate y = x - P/P' and whether its bracket certifies it:
