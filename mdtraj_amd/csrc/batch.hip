@@ -78,6 +78,10 @@ __global__ void ticket_reset_kernel(unsigned* __restrict__ ticket) {
     if (threadIdx.x == 0) *ticket = 0u;
 }
 
+// One workgroup per system.  Its loads go out kPrepBatch at a time per thread
+// (ref_prep_body): the node took 13.6 us per launch on the 512-system mix
+// with one load in flight per thread, 10.1 us with eight; 1,024-thread
+// workgroups were slower (12.5 us).
 __global__ __launch_bounds__(kBlock) void batch_ref_prep_kernel(const DevSystem* __restrict__ sys) {
     const DevSystem s = sys[blockIdx.x];
     if (s.n_frames == 0) return;  // block-uniform: no barrier is skipped by part of it
@@ -432,12 +436,15 @@ int capture_plan(hr_batch_plan* p) {
         }
     } else if (sched == 3) {
         // the largest reference, up to what leaves room for kMinWaves
-        // workgroups per CU.  A workgroup's LDS is allocated in granules:
-        // 54,576 B passes the occupancy query for three per CU on MI355X,
-        // yet a third of such a grid waits for a second round, so the share
-        // is rounded down to whole KiB first.
+        // workgroups per CU.  A workgroup's LDS is allocated in granules
+        // coarser than the occupancy query assumes: on MI355X 54,240 B passes
+        // the query for three per CU, yet only two are resident and a third
+        // of the grid starts when the first ones end (~200 us late on the
+        // 512-system mix: 0.507 ms against 0.449 ms); 53,232 B (4,436
+        // atoms) keeps all three resident.  The share is rounded down to
+        // 2 KiB.
         const size_t per_group =
-            std::min(d->lds_per_cu / kMinWaves, d->lds_per_block) / 1024 * 1024;
+            std::min(d->lds_per_cu / kMinWaves, d->lds_per_block) / 2048 * 2048;
         const long long fit = (long long)per_group / 12 / 4 * 4;
         p->lds_atoms = (int)std::min<long long>(p->max_atoms, fit);
         for (;;) {
