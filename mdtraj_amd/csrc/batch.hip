@@ -1,30 +1,33 @@
 // Batched RMSD over many independent systems of uneven size in one graph
 // launch (gfx950).
 //
-// A plan holds, in device memory, one descriptor per system and a table of
-// work units (system, up to 64 consecutive frames) sorted by cost (frames x
-// atoms, largest first).  Its hipGraph has three nodes on one stream:
-//   1. a one-thread fill kernel that zeroes the unit ticket (no memset node),
-//   2. reference prep: one workgroup per system gathers its reference frame
-//      into axis-major planes and sums it in FP64,
-//   3. the persistent scoring kernel: one workgroup per resident slot; wave w
-//      takes unit w first and then the next unit from the ticket (vector
-//      atomic, the next one requested while the current unit runs) until the
-//      table is exhausted, and runs the per-frame code on it (16 lanes per
-//      frame, FP64 sums, the reference streamed through the load ring with
-//      the frames and passed between lanes by permutes, QCP one lane per
-//      frame).  Taking the first units without the ticket matters: when
-//      every wave of the grid asks one counter at launch, the atomics to that
-//      address serialise and the last waves start a third of a millisecond
-//      late (100,000 x 2,000 in 33-frame units: 0.77 ms against 0.49 ms).
-// Every wave leaves when its unit index passes the table's end, so the grid
-// drains on every launch.  A plan must not be launched on two streams at once
-// (the ticket is shared).
-//
-// HR_TUNE_BATCH_SCHEDULE = 2 replaces the ticket by a static split (nodes 2
-// and 3 only): the frames of all systems form one sequence and each wave
-// takes one contiguous, cost-balanced range of it, like the one-vs-all
-// kernel on a single trajectory, the reference riding the load ring.
+// A plan holds, in device memory, one descriptor per system.  Its hipGraph
+// runs, on one stream, a reference prep kernel (one workgroup per system
+// gathers its reference frame into axis-major planes and sums it in FP64)
+// and then one scoring kernel, chosen by HR_TUNE_BATCH_SCHEDULE:
+//   3 (default) batch_block_kernel: the frames of all systems form one
+//     sequence, each frame costed as its bytes plus a fixed per-frame term
+//     and each system as a fixed start-up term; workgroup g of G (three per
+//     CU) owns the frames starting in its 1/G of the total cost.  For each
+//     system in its range it stages the reference in LDS and its four waves
+//     split the system's frames; frame sums stay in registers.
+//   4 batch_cu_kernel: the same split with one workgroup per CU; the CU's
+//     waves take a static share of each system's frames and then chunks from
+//     an LDS counter.
+//   1 batch_kernel: work units (system, up to 64 consecutive frames) sorted
+//     by cost, largest first; wave w takes unit w first and then the next
+//     unit from a ticket (vector atomic, requested while the current unit
+//     runs), the reference streamed through the load ring with the frames.
+//     A one-thread fill kernel zeroes the ticket first (no memset node).
+//     Taking the first units without the ticket matters: when every wave of
+//     the grid asks one counter at launch, the atomics to that address
+//     serialise and the last waves start a third of a millisecond late
+//     (100,000 x 2,000 in 33-frame units: 0.77 ms against 0.49 ms).  A plan
+//     of this schedule must not be launched on two streams at once (the
+//     ticket is shared).
+//   2 batch_static_kernel: one contiguous cost-balanced range per wave, the
+//     reference read from global memory with the frames.
+// Every kernel's grid drains on every launch.
 #include <algorithm>
 #include <vector>
 
