@@ -44,6 +44,13 @@ const DeviceInfo* current_device();
 // Process-wide tuning knobs (HR_TUNE_*), set by hr_set_tuning; 0 = default.
 int tuning(int knob);
 
+// One-vs-all over contiguous frames of n_sel atoms against a reference
+// selection given as n_sel interleaved atoms, in one launch: the per-frame
+// kernel prepares the reference itself (one_vs_all.hip).  *done is false, and
+// nothing is launched, when the selection does not fit that kernel's LDS.
+int one_vs_all_fused_ref(hipStream_t s, const float* frames, long long n_frames, int n_sel,
+                         const float* ref, long long self_frame, float* out, bool* done);
+
 // Launch status of the last kernel launch.
 inline int launch_status(const char* what) {
     hipError_t e = hipGetLastError();

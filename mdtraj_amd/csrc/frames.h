@@ -53,25 +53,25 @@ constexpr int kStageNone = 0;
 constexpr int kStageIdx = 1;
 constexpr int kStageAll = 2;
 
-// Gather the reference selection into axis-major planes and sum it in FP64
-// (one workgroup of kThreads threads).  Each thread loads kPrepBatch atoms
-// before it uses any (indices clamped into the selection, so no load sits
-// under a branch), so a reference of up to kPrepBatch * kThreads atoms costs
-// one load latency per thread.  The per-thread order of the sums is that of
-// a plain strided loop, so the results do not depend on kPrepBatch.
+// Gather the reference selection into axis-major planes and sum it in FP64.
+// The first kBlock threads of the workgroup (which may be larger) do the
+// work, so the sums come out the same in any kernel that calls it.  Each
+// thread loads kPrepBatch atoms before it uses any (indices clamped into the
+// selection, so no load sits under a branch); the per-thread order of the
+// sums is that of a plain strided loop.
 constexpr int kPrepBatch = 8;
 
-template <int kThreads = kBlock>
 __device__ __forceinline__ void ref_prep_body(const float* __restrict__ ref,
                                               const int* __restrict__ ref_idx, int n_sel,
                                               int axis_rowstride, float* __restrict__ ref_out,
                                               double* __restrict__ stats) {
     double sx = 0.0, sy = 0.0, sz = 0.0, q = 0.0;
-    for (int j0 = threadIdx.x; j0 < n_sel; j0 += kPrepBatch * kThreads) {
+    const int t0 = threadIdx.x < kBlock ? (int)threadIdx.x : n_sel;  // the rest: no atoms
+    for (int j0 = t0; j0 < n_sel; j0 += kPrepBatch * kBlock) {
         float x[kPrepBatch], y[kPrepBatch], z[kPrepBatch];
 #pragma unroll
         for (int u = 0; u < kPrepBatch; ++u) {
-            const int j = min(j0 + u * kThreads, n_sel - 1);
+            const int j = min(j0 + u * kBlock, n_sel - 1);
             if (axis_rowstride > 0) {
                 x[u] = ref[j];
                 y[u] = ref[axis_rowstride + j];
@@ -85,7 +85,7 @@ __device__ __forceinline__ void ref_prep_body(const float* __restrict__ ref,
         }
 #pragma unroll
         for (int u = 0; u < kPrepBatch; ++u) {
-            const int j = j0 + u * kThreads;
+            const int j = j0 + u * kBlock;
             if (j < n_sel) {
                 ref_out[j] = x[u];
                 ref_out[n_sel + j] = y[u];
@@ -102,10 +102,10 @@ __device__ __forceinline__ void ref_prep_body(const float* __restrict__ ref,
     sy = wave_sum(sy);
     sz = wave_sum(sz);
     q = wave_sum(q);
-    __shared__ double part[kThreads / kWave][4];
+    __shared__ double part[kWavesPerBlock][4];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    if (lane == 0) {
+    if (lane == 0 && w < kWavesPerBlock) {
         part[w][0] = sx;
         part[w][1] = sy;
         part[w][2] = sz;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void ref_prep_body(const float* __restrict__ ref,
     __syncthreads();
     if (threadIdx.x < 4) {
         double s = 0.0;
-        for (int i = 0; i < kThreads / kWave; ++i) s += part[i][threadIdx.x];
+        for (int i = 0; i < kWavesPerBlock; ++i) s += part[i][threadIdx.x];
         stats[threadIdx.x] = s;
     }
 }

@@ -37,12 +37,6 @@ constexpr size_t kChunkBytes = 64u << 20;     // gathered bytes per chunk
 constexpr size_t kParallelBytes = 1u << 20;   // below this, gather on the calling thread
 constexpr size_t kDirectBytes = 1u << 20;     // default HR_TUNE_HOST_DIRECT_KIB
 
-// Device-visible address of pinned host memory.
-int device_view(void* host, void** dev) {
-    HR_CHECK_HIP(hipHostGetDevicePointer(dev, host, 0));
-    return HR_OK;
-}
-
 // Fixed set of workers that run one function over [0, n) indices at a time.
 class WorkerPool {
 public:
@@ -105,6 +99,7 @@ struct Buffer {
     void* p = nullptr;
     size_t bytes = 0;
     bool pinned = false;
+    void* dev = nullptr;  // pinned: the device-visible address, looked up once
 };
 
 int grow(Buffer* b, size_t need, bool pinned) {
@@ -113,6 +108,7 @@ int grow(Buffer* b, size_t need, bool pinned) {
         hipError_t e = b->pinned ? hipHostFree(b->p) : hipFree(b->p);
         b->p = nullptr;
         b->bytes = 0;
+        b->dev = nullptr;
         if (e != hipSuccess) {
             set_error("freeing a staging buffer failed: %s", hipGetErrorString(e));
             return HR_ERR_HIP;
@@ -128,6 +124,14 @@ int grow(Buffer* b, size_t need, bool pinned) {
     }
     b->bytes = need;
     b->pinned = pinned;
+    b->dev = nullptr;
+    if (pinned) {
+        e = hipHostGetDevicePointer(&b->dev, b->p, 0);
+        if (e != hipSuccess) {
+            set_error("hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
+            return HR_ERR_HIP;
+        }
+    }
     return HR_OK;
 }
 
@@ -174,18 +178,20 @@ void gather_frames(HostContext* c, const float* target, int64_t n_atoms_total,
                    const int32_t* idx, int32_t n_sel, int64_t f0, int64_t f1, float* dst) {
     const size_t frame_out = (size_t)n_sel * 3;
     auto range = [&](int64_t a, int64_t b) {
+        if (!idx) {
+            // whole frames: the range is one contiguous block
+            std::memcpy(dst + (size_t)(a - f0) * frame_out, target + (size_t)a * frame_out,
+                        (size_t)(b - a) * frame_out * sizeof(float));
+            return;
+        }
         for (int64_t f = a; f < b; ++f) {
             const float* src = target + (size_t)f * n_atoms_total * 3;
             float* o = dst + (size_t)(f - f0) * frame_out;
-            if (!idx) {
-                std::memcpy(o, src, frame_out * sizeof(float));
-            } else {
-                for (int32_t i = 0; i < n_sel; ++i) {
-                    const float* a3 = src + (size_t)idx[i] * 3;
-                    o[3 * i] = a3[0];
-                    o[3 * i + 1] = a3[1];
-                    o[3 * i + 2] = a3[2];
-                }
+            for (int32_t i = 0; i < n_sel; ++i) {
+                const float* a3 = src + (size_t)idx[i] * 3;
+                o[3 * i] = a3[0];
+                o[3 * i + 1] = a3[1];
+                o[3 * i + 2] = a3[2];
             }
         }
     };
@@ -232,17 +238,22 @@ int one_vs_all_direct(HostContext* c, const float* target, int64_t n_frames,
     gather_reference(c, ref_frame, ref_idx, n_sel);
     gather_frames(c, target, n_atoms_total, atom_idx, n_sel, 0, n_frames,
                   static_cast<float*>(c->pinned_slot[0].p));
-    void *d_ref = nullptr, *d_frames = nullptr, *d_out = nullptr;
-    if ((rc = device_view(c->pinned_ref.p, &d_ref))) return rc;
-    if ((rc = device_view(c->pinned_slot[0].p, &d_frames))) return rc;
-    if ((rc = device_view(c->pinned_out.p, &d_out))) return rc;
-    if ((rc = hr_rmsd_prepare_reference(static_cast<const float*>(d_ref), nullptr, n_sel,
-                                        c->dev_ws.p, c->stream)))
+    const float* d_ref = static_cast<const float*>(c->pinned_ref.dev);
+    const float* d_frames = static_cast<const float*>(c->pinned_slot[0].dev);
+    float* d_out = static_cast<float*>(c->pinned_out.dev);
+    // one launch when the reference fits the per-frame kernel's LDS (the
+    // kernel prepares it with the prep kernel's arithmetic: the same bits)
+    bool done = false;
+    if ((rc = one_vs_all_fused_ref(c->stream, d_frames, n_frames, n_sel, d_ref, self_frame, d_out,
+                                   &done)))
         return rc;
-    if ((rc = hr_rmsd_one_vs_all_prepared(static_cast<const float*>(d_frames), n_frames, n_sel,
-                                          nullptr, n_sel, self_frame, static_cast<float*>(d_out),
-                                          c->dev_ws.p, c->stream)))
-        return rc;
+    if (!done) {
+        if ((rc = hr_rmsd_prepare_reference(d_ref, nullptr, n_sel, c->dev_ws.p, c->stream)))
+            return rc;
+        if ((rc = hr_rmsd_one_vs_all_prepared(d_frames, n_frames, n_sel, nullptr, n_sel,
+                                              self_frame, d_out, c->dev_ws.p, c->stream)))
+            return rc;
+    }
     HR_CHECK_HIP(hipStreamSynchronize(c->stream));
     std::memcpy(out, c->pinned_out.p, (size_t)n_frames * sizeof(float));
     return HR_OK;

@@ -50,6 +50,10 @@ __global__ __launch_bounds__(CuGroup<kMinWaves>::kThreads, kMinWaves) void frame
     const float* ry = ref_sel + n_sel;
     const float* rz = ref_sel + 2 * n_sel;
     const int* ix = idx;
+    // ref_stats == nullptr (Kind::Rmsd, kStageAll only): ref_sel is the
+    // reference selection itself (n_sel atoms, xyz interleaved), and every
+    // workgroup prepares it, with the prep kernel's arithmetic, into its LDS
+    __shared__ double s_stats[4];
     if (threadIdx.x == 0) s_ticket = 0u;
     if constexpr (kStage != kStageNone) {
         // LDS after the result area: [reference planes (kStageAll)] [indices]
@@ -57,7 +61,12 @@ __global__ __launch_bounds__(CuGroup<kMinWaves>::kThreads, kMinWaves) void frame
         float* s_ry = smem + n_sel;
         float* s_rz = smem + 2 * n_sel;
         int* s_idx = reinterpret_cast<int*>(kStage == kStageAll ? smem + 3 * n_sel : smem);
-        if constexpr (kStage == kStageAll) stage_words<kThreads>(s_rx, ref_sel, 3 * n_sel);
+        if constexpr (kStage == kStageAll) {
+            if (kKind == Kind::Rmsd && ref_stats == nullptr)  // uniform over the grid
+                ref_prep_body(ref_sel, nullptr, n_sel, 0, s_rx, s_stats);
+            else
+                stage_words<kThreads>(s_rx, ref_sel, 3 * n_sel);
+        }
         if constexpr (kGather) stage_words<kThreads>(s_idx, idx, n_sel);
         if constexpr (kStage == kStageAll) {
             rx = s_rx;
@@ -69,10 +78,14 @@ __global__ __launch_bounds__(CuGroup<kMinWaves>::kThreads, kMinWaves) void frame
     __syncthreads();
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
     if constexpr (kKind != Kind::NoSuperpose) {
-        rs[0] = ref_stats[0];
-        rs[1] = ref_stats[1];
-        rs[2] = ref_stats[2];
-        rs[3] = ref_stats[3];
+        const double* st = ref_stats;
+        if constexpr (kStage == kStageAll && kKind == Kind::Rmsd) {
+            if (ref_stats == nullptr) st = s_stats;
+        }
+        rs[0] = st[0];
+        rs[1] = st[1];
+        rs[2] = st[2];
+        rs[3] = st[3];
     }
     // whole frame sets (a set fills the wave's kWave / kRL rows): the
     // workgroup's sets b0 .. b1, the first n_static of them split evenly
@@ -233,7 +246,12 @@ int launch_frames_v(const Launch& l, hipStream_t s, const float* xyz, long long 
         lds_attr_set = true;
     }
     if ((rc = frames_grid(k, kThreads, l.lds, n_frames, kRows, &blocks))) return rc;
+    // With at most one frame set per wave there is nothing to balance: every
+    // set is assigned statically, so that no wave takes a chunk of several
+    // sets in series while others idle (1,000 frames x 22 atoms: 18 us).
+    const bool one_each = (n_frames + kRows - 1) / kRows <= (long long)blocks * (kThreads / kWave);
     const int pct = tuning(HR_TUNE_FRAMES_STATIC_PCT) ? tuning(HR_TUNE_FRAMES_STATIC_PCT)
+                    : one_each                        ? 100
                                                       : kDefaultStaticPct;
     const long long kib = tuning(HR_TUNE_FRAMES_CHUNK_KIB) ? tuning(HR_TUNE_FRAMES_CHUNK_KIB)
                                                           : kDefaultChunkKiB;
@@ -277,6 +295,25 @@ int launch_frames(const Launch& l, hipStream_t s, const float* xyz, long long n_
 #undef HR_FRAMES_RL
 #undef HR_FRAMES_ARGS
 }
+
+}  // namespace
+
+// One launch for a small call: the per-frame kernel prepares the reference
+// itself (see frames_kernel).  Returns HR_OK with *done false, launching
+// nothing, when the selection does not fit the kernel's LDS whole.
+int one_vs_all_fused_ref(hipStream_t s, const float* frames, long long n_frames, int n_sel,
+                         const float* ref, long long self_frame, float* out, bool* done) {
+    *done = false;
+    Launch l;
+    int rc = plan<Kind::Rmsd>(n_sel, false, &l);
+    if (rc || l.stage != kStageAll) return rc;
+    *done = true;
+    return launch_frames<Kind::Rmsd, false, false>(l, s, frames, n_frames, 3LL * n_sel, 0,
+                                                   nullptr, n_sel, ref, nullptr, self_frame, out,
+                                                   nullptr, nullptr);
+}
+
+namespace {
 
 int prep_ref(hipStream_t s, const float* ref, const int* ref_idx, int n_sel, int axis_rowstride,
              void* workspace, float** ref_sel, double** stats) {
