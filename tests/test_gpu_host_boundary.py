@@ -68,6 +68,28 @@ def test_direct_and_copied_staging_agree(gpu, F, N, n_sel):
     assert got[0][3] == 0.0
 
 
+@pytest.mark.parametrize("F", [1, 2, 15, 16, 17, 63, 64, 65, 191, 193, 1000])
+@pytest.mark.parametrize("N", [3, 22, 150])
+def test_single_launch_small_calls(gpu, F, N):
+    """Small host calls run one kernel that prepares the reference itself and
+    assigns every frame set statically when there is at most one per wave;
+    frame counts around the row and wave boundaries give the same bits as the
+    copied path (reference prep kernel, then the per-frame kernel)."""
+    x = gen.walk(seed=F * 7 + N, n_frames=F, n_atoms=N)
+    t = md.Trajectory(x)
+    frame = F // 2
+    got = {}
+    try:
+        for kib in (0, -1):
+            _lib.call("hr_set_tuning", _lib.HR_TUNE_HOST_DIRECT_KIB, kib)
+            got[kib] = md.rmsd(t, t, frame)
+    finally:
+        _lib.call("hr_set_tuning", _lib.HR_TUNE_HOST_DIRECT_KIB, 0)
+    assert np.array_equal(got[0], got[-1])
+    assert got[0][frame] == 0.0
+    assert np.abs(got[0] - fp64.rmsd(x, x, frame)).max() < TOL_FP64
+
+
 def test_host_entry_ref_indices_and_other_reference(gpu):
     x = gen.walk(seed=5, n_frames=400, n_atoms=120)
     r = gen.walk(seed=6, n_frames=3, n_atoms=150)
