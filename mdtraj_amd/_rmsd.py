@@ -31,6 +31,7 @@ memoryview does.
 """
 from __future__ import annotations
 
+import ctypes
 import warnings
 
 import numpy as np
@@ -150,15 +151,24 @@ def _rmsd_host(txyz, rxyz, frame, idx, ridx, n_sel, same):
     chunks and scores each chunk on the GPU as it arrives."""
     dev.require_gpu()
     T = np.ascontiguousarray(txyz, dtype=np.float32)
-    R = np.ascontiguousarray(np.asarray(rxyz)[frame], dtype=np.float32)
     n_frames, n_atoms = T.shape[0], T.shape[1]
-    out = np.empty(n_frames, np.float32)
     if n_frames == 0:
-        return out
+        return np.empty(0, np.float32)
+    t_ptr = T.ctypes.data
+    if same:
+        # the reference frame is frame `frame` of the same (converted) array
+        r_ptr, n_ref_atoms = t_ptr + frame * 12 * n_atoms, n_atoms
+    else:
+        R = np.ascontiguousarray(np.asarray(rxyz)[frame], dtype=np.float32)
+        r_ptr, n_ref_atoms = R.ctypes.data, R.shape[0]
+    # the result's buffer comes from ctypes, whose address is at hand (an
+    # ndarray's .ctypes.data costs about a microsecond per call)
+    buf = (ctypes.c_float * n_frames)()
+    out = np.frombuffer(buf, dtype=np.float32)
     a, r = _i32_host(idx), _i32_host(ridx)
-    call("hr_rmsd_one_vs_all_host", T.ctypes.data, n_frames, n_atoms,
-         None if a is None else a.ctypes.data, n_sel, R.ctypes.data, R.shape[0],
-         None if r is None else r.ctypes.data, frame if same else -1, out.ctypes.data)
+    call("hr_rmsd_one_vs_all_host", t_ptr, n_frames, n_atoms,
+         None if a is None else a.ctypes.data, n_sel, r_ptr, n_ref_atoms,
+         None if r is None else r.ctypes.data, frame if same else -1, ctypes.addressof(buf))
     return out
 
 
