@@ -38,6 +38,13 @@ __device__ __forceinline__ float round_half(float x) {
 }
 
 // Gather + centre + round + trace, one wave per frame; split planes too.
+// Frames of up to kPrepRegs * 64 padded atoms are loaded into registers in
+// one batch (every load issued before any is used) and the three passes
+// read them there; longer frames are read from memory in each pass.  Either
+// way each lane sums its atoms j = lane, lane + 64, ... in that order, so the
+// results do not depend on the path.
+constexpr int kPrepRegs = 8;
+
 template <typename Half, Split kSplit>
 __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
     const float* __restrict__ xyz, long long n_frames, int n_atoms_total,
@@ -47,20 +54,44 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (kPrepBlock / kWave) + threadIdx.x / kWave;
     const long long n_waves = (long long)gridDim.x * (kPrepBlock / kWave);
+    const bool in_regs = k_pad <= kPrepRegs * kWave;  // uniform over the grid
     for (long long f = wave; f < n_frames; f += n_waves) {
         const float* fr = xyz + f * 3LL * n_atoms_total;
+        float rx[kPrepRegs], ry[kPrepRegs], rz[kPrepRegs];
+        if (in_regs) {
+#pragma unroll
+            for (int u = 0; u < kPrepRegs; ++u) {
+                const int jc = min(lane + u * kWave, n_sel - 1);
+                const long long a = idx ? idx[jc] : jc;
+                rx[u] = fr[3 * a];
+                ry[u] = fr[3 * a + 1];
+                rz[u] = fr[3 * a + 2];
+            }
+        }
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int j = lane; j < n_sel; j += kWave) {
-            const long long a = idx ? idx[j] : j;
-            sx += (double)fr[3 * a];
-            sy += (double)fr[3 * a + 1];
-            sz += (double)fr[3 * a + 2];
+        if (in_regs) {
+#pragma unroll
+            for (int u = 0; u < kPrepRegs; ++u) {
+                if (lane + u * kWave < n_sel) {
+                    sx += (double)rx[u];
+                    sy += (double)ry[u];
+                    sz += (double)rz[u];
+                }
+            }
+        } else {
+            for (int j = lane; j < n_sel; j += kWave) {
+                const long long a = idx ? idx[j] : j;
+                sx += (double)fr[3 * a];
+                sy += (double)fr[3 * a + 1];
+                sz += (double)fr[3 * a + 2];
+            }
         }
         sx = wave_sum(sx) / n_sel;
         sy = wave_sum(sy) / n_sel;
         sz = wave_sum(sz) / n_sel;
-        // centred, rounded coordinates of selected atom j (0 past the selection)
-        auto centred_xyz = [&](int j, float& x, float& y, float& z) {
+        // centred, rounded coordinates of selected atom j (0 past the
+        // selection), from memory or from register slot u
+        auto centred_mem = [&](int j, float& x, float& y, float& z) {
             x = y = z = 0.0f;
             if (j < n_sel) {
                 const long long a = idx ? idx[j] : j;
@@ -69,12 +100,38 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
                 z = (float)((double)fr[3 * a + 2] - sz);
             }
         };
+        auto centred_reg = [&](int j, int u, float& x, float& y, float& z) {
+            x = y = z = 0.0f;
+            if (j < n_sel) {
+                x = (float)((double)rx[u] - sx);
+                y = (float)((double)ry[u] - sy);
+                z = (float)((double)rz[u] - sz);
+            }
+        };
+        // runs body(j, x, y, z) over the padded atoms j = lane, lane + 64, ...
+        auto each_atom = [&](auto&& body) {
+            if (in_regs) {
+#pragma unroll
+                for (int u = 0; u < kPrepRegs; ++u) {
+                    const int j = lane + u * kWave;
+                    if (j < k_pad) {
+                        float x, y, z;
+                        centred_reg(j, u, x, y, z);
+                        body(j, x, y, z);
+                    }
+                }
+            } else {
+                for (int j = lane; j < k_pad; j += kWave) {
+                    float x, y, z;
+                    centred_mem(j, x, y, z);
+                    body(j, x, y, z);
+                }
+            }
+        };
         float* px = planes ? planes + f * 3LL * k_pad : nullptr;
         double g = 0.0;
         float m = 0.0f;
-        for (int j = lane; j < k_pad; j += kWave) {
-            float x, y, z;
-            centred_xyz(j, x, y, z);
+        each_atom([&](int j, float x, float y, float z) {
             if (planes) {
                 px[j] = x;
                 px[k_pad + j] = y;
@@ -83,7 +140,7 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
             const double dx = x, dy = y, dz = z;
             g = fma(dx, dx, fma(dy, dy, fma(dz, dz, g)));
             m = fmaxf(m, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
-        }
+        });
         g = wave_sum(g);
         if (lane == 0 && traces) traces[f] = g;
         if (split == nullptr) continue;
@@ -97,9 +154,8 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
         const double inv = ldexp(1.0, -sh);
         double gs = 0.0;
         Half* row = split + f * 6LL * k_pad;  // this frame's k_pad / 32 chunks
-        for (int j = lane; j < k_pad; j += kWave) {
-            float c[3];
-            centred_xyz(j, c[0], c[1], c[2]);
+        each_atom([&](int j, float x, float y, float z) {
+            const float c[3] = {x, y, z};
             Half* chunk = row + (j / kChunk) * (6 * kChunk) + (j % kChunk);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -111,7 +167,7 @@ __global__ __launch_bounds__(kPrepBlock) void pairs_prep_kernel(
                 const double v = ((double)hf + (double)lf) * inv;
                 gs = fma(v, v, gs);
             }
-        }
+        });
         gs = wave_sum(gs);
         if (lane == 0) {
             scale[f] = ldexpf(1.0f, -sh);
