@@ -143,6 +143,30 @@ def test_walk_pairs_stay_on_the_newton_path(qcp):
             assert qcp.qcp_newton_resolved_host(M.ctypes.data_as(_P), ga, gb) == 1
 
 
+def test_isotropic_structures_where_the_start_bound_is_tight(qcp):
+    """A cube's corners have isotropic inertia: against a rotated copy, M = c R
+    has three equal singular values, so sqrt(3) |M|_F (one of the two upper
+    bounds Newton starts from) is lambda_max itself.  The start, widened by
+    1e-6, still lies above every root: the solve stays on the Newton path and
+    matches the eigensolver.  Against an inverted copy (M = -c R) the top
+    eigenvalue of K is triple (every half-turn fits equally well): that root
+    takes the Jacobi fallback, and the RMSD still matches."""
+    corners = 0.7 * np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)],
+                             np.float64)
+    rng = np.random.default_rng(11)
+    R = gen.rotation_matrix_from_quaternion(gen.uniform_quaternion(rng, 40))
+    for k in range(40):
+        for sign in (1.0, -1.0):
+            a = (corners + rng.uniform(-2, 2, 3)).astype(np.float32)
+            b = (sign * corners @ R[k].T + rng.uniform(-2, 2, 3)).astype(np.float32)
+            M, ga, gb, n = _sums(a, b)
+            resolved = qcp.qcp_newton_resolved_host(M.ctypes.data_as(_P), ga, gb)
+            assert resolved == (1 if sign > 0 else 0), (k, sign)
+            got = np.sqrt(qcp.qcp_msd_host(M.ctypes.data_as(_P), ga, gb, float(n)))
+            want = fp64.rmsd(a[None], b[None], 0)[0]
+            assert abs(got - want) < 2e-6, (k, sign, got, want)
+
+
 def test_single_atom_and_zero(qcp):
     for a in (np.zeros((1, 3), np.float32), np.ones((1, 3), np.float32),
               np.zeros((5, 3), np.float32)):
